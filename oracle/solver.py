@@ -138,6 +138,42 @@ def assemble_ks_level2_super(upMap, Ks0, Psub, mapUnique, eleModulus, uniqueFree
     return Ks
 
 
+def assemble_ks_level2_fast(upMap, eleModulus, K0col, Pch, mapUnique=None, uniqueFree=None, uniqueFixed=None):
+    """Vectorised assemble_ks_level2 for large meshes: Ks_e = sum_s E_f(e,s) * (P_s' K0 P_s) plus
+    the boundary children's corrections P_s' (E (Ufree - K0) + Ufixed) P_s.  Equal to the literal
+    loop up to rounding (tests/test_oracle_solver.py pins it)."""
+    Ne, nCh = upMap.shape
+    K0 = np.asarray(K0col).reshape(24, 24, order='F')
+    G = np.einsum('kis,kl,ljs->sij', Pch, K0, Pch)
+    up = upMap.astype(np.int64)
+    Ech = np.where(up > 0, eleModulus[np.maximum(up, 1) - 1], 0.0)
+    Ks = np.ascontiguousarray((Ech @ G.reshape(nCh, 576)).T.reshape(24, 24, Ne))
+    if uniqueFree is not None and uniqueFree.size > 0 and mapUnique is not None:
+        es, ss = np.nonzero(up > 0)
+        f = up[es, ss] - 1
+        uid = mapUnique[f].astype(np.int64) - 1
+        sel = uid >= 0
+        for e, s, ff, u in zip(es[sel], ss[sel], f[sel], uid[sel]):
+            D = ((uniqueFree[:, u] - K0.reshape(-1, order='F')) * eleModulus[ff] + uniqueFixed[:, u]).reshape(
+                24, 24, order='F')
+            B = Pch[:, :, s]
+            Ks[:, :, e] += B.T @ D @ B
+    return Ks
+
+
+def assemble_ks_higher_level_fast(upMap, KsPrev, Psub):
+    """Vectorised assemble_ks_higher_level (batched matmuls per child slot)."""
+    Ne, nSub = upMap.shape
+    out = np.zeros((Ne, 24, 24))
+    Kp = np.moveaxis(KsPrev, 2, 0)
+    for s in range(nSub):
+        f = upMap[:, s].astype(np.int64)
+        ok = f > 0
+        B = Psub[:, :, s]
+        out[ok] += B.T @ Kp[f[ok] - 1] @ B
+    return np.moveaxis(out, 0, 2).copy()
+
+
 # ---------------------------------------------------------------------------------------
 # Solver state (the MATLAB globals the hot path reads)
 # ---------------------------------------------------------------------------------------
@@ -247,10 +283,11 @@ class Solver:
         return x[0]
 
     # --- assembly -----------------------------------------------------------------------
-    def assemble(self, eleModulus):
+    def assemble(self, eleModulus, fast=None):
         """Solving_AssembleFEAstencil (TOP_GPU.m:1033-1267) for the current moduli.
 
-        eleModulus: [Ne] (standard) or [64 x Ne] (super-element mode).
+        eleModulus: [Ne] (standard) or [64 x Ne] (super-element mode).  fast selects the
+        vectorised Galerkin products (default: on for meshes with more than 4096 coarse elements).
         """
         m = self.m
         lv1 = self.levels[0]
@@ -259,16 +296,19 @@ class Solver:
         for ii in range(2, self.L + 1):
             lv = self.levels[ii - 1]
             P = build_pch(lv.spanWidth)
+            vec = fast if fast is not None else lv.numElements > 4096
             if ii == 2:
                 if not m.super_element:
-                    lv.Ks = assemble_ks_level2(lv.elementUpwardMap, lv1.eleModulus, lv1.Ke.reshape(-1, order='F'),
-                                               P, m.mapUniqueKes, m.uniqueKesFree, m.uniqueKesFixed)
+                    f = assemble_ks_level2_fast if vec else assemble_ks_level2
+                    lv.Ks = f(lv.elementUpwardMap, lv1.eleModulus, lv1.Ke.reshape(-1, order='F'),
+                              P, m.mapUniqueKes, m.uniqueKesFree, m.uniqueKesFixed)
                 else:
                     self.K_mat = m.I_mat @ lv1.eleModulus
                     lv.Ks = assemble_ks_level2_super(lv.elementUpwardMap, self.K_mat, P, m.mapUniqueKes,
                                                      lv1.eleModulus, m.uniqueKesFree, m.uniqueKesFixed)
             else:
-                lv.Ks = assemble_ks_higher_level(lv.elementUpwardMap, self.levels[ii - 2].Ks, P)
+                f = assemble_ks_higher_level_fast if vec else assemble_ks_higher_level
+                lv.Ks = f(lv.elementUpwardMap, self.levels[ii - 2].Ks, P)
             if ii < self.L:
                 blk = lv.Ks.reshape(576, -1, order='F')[0::25, :].T                  # [Ne x 24]
                 dk = np.zeros((lv.numNodes, 3))

@@ -1,0 +1,109 @@
+"""GPU parity of the device solver (libtopgpu, through the C ABI) against the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import mesh as M
+from oracle import solver as S
+from tests.conftest import holey_mask, random_modulus
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    'box': dict(shape=(8, 16, 8), case=1, vox=None, se=False),
+    'holes': dict(shape=(12, 20, 12), case=1, vox=holey_mask(12, 20, 12, seed=3), se=False),
+    'mbb_super': dict(shape=(10, 20, 10), case=2, vox=None, se=True),
+}
+
+
+def _rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.fixture(scope='module', params=list(CASES))
+def pair(request):
+    import torch
+    from top_gpu_on_matlab_amd.model import Model
+    from top_gpu_on_matlab_amd.solver import Solver
+    c = CASES[request.param]
+    ny, nx, nz = c['shape']
+    print('\n[case %s shape %s]' % (request.param, c['shape']), flush=True)
+    vox = c['vox'] if c['vox'] is not None else np.ones(c['shape'], bool)
+    om = M.build_cuboid_model(ny, nx, nz, opt_case=c['case'], voxels=vox, super_element=c['se'])
+    gm = Model(vox, super_element=c['se'])
+    gm.builtin_case(c['case'])
+    gm.finalize()
+    Ne = om.levels[0].numElements
+    E = random_modulus(Ne * (64 if c['se'] else 1), seed=2)
+    if c['se']:
+        E = E.reshape(64, Ne, order='F')
+    osv = S.Solver(om)
+    osv.assemble(E)
+    gsv = Solver(gm)
+    gsv.set_modulus(np.asarray(E).reshape(-1, order='F'))
+    gsv.assemble()
+    torch.cuda.synchronize()
+    return request.param, om, osv, gm, gsv, E
+
+
+def test_kbyu_all_levels(pair):
+    name, om, osv, gm, gsv, E = pair
+    rng = np.random.default_rng(1)
+    for level in range(1, om.numLevels + 1):
+        n = 3 * om.levels[level - 1].numNodes
+        u = rng.standard_normal(n)
+        ref = osv.kbyu(u, level)
+        got = gsv.kbyu(u, level).cpu().numpy()
+        assert _rel(got, ref) < 1e-13, (name, level, _rel(got, ref))
+
+
+def test_transfer_operators(pair):
+    name, om, osv, gm, gsv, E = pair
+    rng = np.random.default_rng(2)
+    for ii in range(2, om.numLevels + 1):
+        rf = rng.standard_normal(3 * om.levels[ii - 2].numNodes)
+        xc = rng.standard_normal(3 * om.levels[ii - 1].numNodes)
+        assert _rel(gsv.restrict(rf, ii).cpu().numpy(), osv.restrict(rf, ii)) < 1e-13
+        assert _rel(gsv.interpolate(xc, ii).cpu().numpy(), osv.interpolate(xc, ii)) < 1e-13
+
+
+def test_galerkin_ks_and_diagonals(pair):
+    name, om, osv, gm, gsv, E = pair
+    for level in range(2, om.numLevels + 1):
+        ks = gsv.ks(level).cpu().numpy()                       # [e][row][col]
+        ref = om.levels[level - 1].Ks.transpose(2, 0, 1)
+        assert _rel(ks, ref) < 1e-13, (name, level)
+    for level in range(1, om.numLevels):
+        d = gsv.diag(level).cpu().numpy()
+        assert _rel(d, om.levels[level - 1].diagK) < 1e-13, (name, level)
+
+
+def test_coarse_solve(pair):
+    name, om, osv, gm, gsv, E = pair
+    r = np.random.default_rng(3).standard_normal(3 * om.levels[-1].numNodes)
+    assert _rel(gsv.coarse_solve(r).cpu().numpy(), osv.coarse_solve(r)) < 1e-9
+
+
+def test_vcycle(pair):
+    name, om, osv, gm, gsv, E = pair
+    r = np.random.default_rng(4).standard_normal(3 * om.levels[0].numNodes)
+    assert _rel(gsv.vcycle(r).cpu().numpy(), osv.vcycle(r)) < 1e-9
+
+
+def test_pcg_parity_and_determinism(pair):
+    name, om, osv, gm, gsv, E = pair
+    b = om.F[:, 0]
+    y_ref, its_ref, hist = osv.pcg(b, tol=1e-3, maxIT=800)
+    y, its, res = gsv.pcg(b, tol=1e-3, max_it=800)
+    y2, its2, res2 = gsv.pcg(b, tol=1e-3, max_it=800)
+    assert its == its_ref, (name, its, its_ref, hist[-3:], res)
+    assert _rel(y.cpu().numpy(), y_ref) < 1e-9
+    assert its2 == its and bool((y == y2).all()), 'PCG is not bitwise reproducible'
+    # warm start from the previous solution (TOP_GPU.m:186)
+    y3, its3, _ = gsv.pcg(b, y0=y, tol=1e-3, max_it=800)
+    y3r, its3r, _ = osv.pcg(b, tol=1e-3, maxIT=800, y0=y.cpu().numpy())
+    assert its3 == its3r
+    assert _rel(y3.cpu().numpy(), y3r) < 1e-9

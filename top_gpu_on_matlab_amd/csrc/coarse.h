@@ -1,0 +1,45 @@
+// Coarsest-level direct solver (replaces the host CHOLMOD factorisation and the per-V-cycle
+// host round trip of TOP_GPU.m:796-802, 1242-1256).
+//
+// The coarsest grid is cut into node planes along its longest axis.  With nodes ordered plane
+// by plane, K_c is block tridiagonal; its Cholesky factor is block bidiagonal:
+//   L_pp L_pp^T = A_pp - L_{p,p-1} L_{p,p-1}^T,   L_{p+1,p} = A_{p+1,p} L_pp^{-T}.
+// The factorisation runs on the device (rocSOLVER potrf/trtri, rocBLAS syrk/trsm/gemm).  Each
+// solve is 2P dense GEMV steps with precomputed row-major operators
+//   forward  y_p = [inv(L_pp) | -inv(L_pp) L_{p,p-1}]      * [b_p; y_{p-1}]
+//   backward x_p = [inv(L_pp)^T | -inv(L_pp)^T L_{p+1,p}^T] * [y_p; x_{p+1}].
+#pragma once
+
+#include <rocsolver/rocsolver.h>
+
+#include "kernels.h"
+#include "model.h"
+
+namespace tg {
+
+struct CoarseSolver {
+    int axis = 0;                   // 0 = y, 1 = x, 2 = z
+    int P = 0;                      // number of planes
+    std::vector<int64_t> nd;        // DOFs per plane
+    std::vector<int64_t> off;       // DOF offset per plane
+    std::vector<int64_t> adOff, asOff, wOff, vOff;  // offsets into the dense buffers
+    int64_t ntot = 0;
+    Grid g{};
+    DevBuf<int32_t> lidx;           // per full-grid node: local node index within its plane (-1 none)
+    DevBuf<int64_t> d_off, d_adOff, d_asOff;
+    DevBuf<double> Ad, As, Linv, Fm, Hm, W, V;
+    DevBuf<double> bb, yy, xx;
+    DevBuf<rocblas_int> info;
+    rocblas_handle h = nullptr;
+    bool factored = false;
+
+    ~CoarseSolver();
+    void setup(const HostLevel &L, const Grid &grid, hipStream_t st);
+    // Assemble the plane blocks from the level's element matrices (entry-major, ldk) and factorise.
+    void factor(const double *Ks, int64_t ldk, const uint8_t *eleMask, const uint8_t *nodeMask, hipStream_t st);
+    // x = K_c^{-1} r on full-grid interleaved vectors (T = float or double; solve in FP64).
+    template <typename T>
+    void solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st);
+};
+
+}  // namespace tg

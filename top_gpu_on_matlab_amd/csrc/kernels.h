@@ -1,0 +1,385 @@
+// Device kernels of the structured-grid solver.
+//
+// Layout: every level keeps its vectors on the level's full padded grid
+// (NY = resY+1, NX = resX+1, NZ = resZ+1 nodes, y fastest), DOF-interleaved: v[3n + c].
+// Nodes outside the mesh hold 0.  A per-node mask byte carries bit c = "DOF c fixed"
+// (finest level) and bit 3 = "node belongs to the mesh".
+//
+// All kernels are node-owned gathers: every output value is written by exactly one lane
+// and summed in a fixed order, so results are bitwise reproducible.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tg {
+
+constexpr uint8_t NODE_ACTIVE = 8;
+
+// eNodMat vertex pattern (dy, dx, dz) and its inverse.
+__host__ __device__ constexpr int vtx_y(int b) { return (b == 0 || b == 1 || b == 4 || b == 5) ? 1 : 0; }
+__host__ __device__ constexpr int vtx_x(int b) { return (b == 1 || b == 2 || b == 5 || b == 6) ? 1 : 0; }
+__host__ __device__ constexpr int vtx_z(int b) { return b >= 4 ? 1 : 0; }
+__host__ __device__ constexpr int vtx_of(int ty, int tx, int tz) { return 4 * tz + (ty ? (tx ? 1 : 0) : (tx ? 2 : 3)); }
+// neighbour slot of relative node offset (dy, dx, dz) in {-1,0,1}^3
+__host__ __device__ constexpr int nb_of(int dy, int dx, int dz) { return (dy + 1) + 3 * (dx + 1) + 9 * (dz + 1); }
+
+struct Grid {
+    int32_t NY, NX, NZ;  // nodes
+    int32_t EY, EX, EZ;  // elements
+    __host__ __device__ int64_t nodes() const { return (int64_t)NY * NX * NZ; }
+    __host__ __device__ int64_t elems() const { return (int64_t)EY * EX * EZ; }
+};
+
+enum { EPI_KU = 0, EPI_RESID = 1, EPI_JACOBI = 2 };
+
+// Load the 27 neighbour nodes' values (fixed DOFs zeroed when MASKED).
+template <typename T, bool MASKED>
+__device__ inline void load_neighbours(T un[27][3], const T *__restrict__ u, const uint8_t *__restrict__ mask,
+                                       const Grid g, int iy, int ix, int iz, int64_t n) {
+#pragma unroll
+    for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int k = nb_of(dy, dx, dz);
+                const bool ok = (unsigned)(iy + dy) < (unsigned)g.NY && (unsigned)(ix + dx) < (unsigned)g.NX &&
+                                (unsigned)(iz + dz) < (unsigned)g.NZ;
+                const int64_t m = n + dy + (int64_t)g.NY * (dx + (int64_t)g.NX * dz);
+                T v0 = 0, v1 = 0, v2 = 0;
+                if (ok) {
+                    v0 = u[3 * m];
+                    v1 = u[3 * m + 1];
+                    v2 = u[3 * m + 2];
+                    if (MASKED) {
+                        const uint8_t mm = mask[m];
+                        if (mm & 1) v0 = 0;
+                        if (mm & 2) v1 = 0;
+                        if (mm & 4) v2 = 0;
+                    }
+                }
+                un[k][0] = v0;
+                un[k][1] = v1;
+                un[k][2] = v2;
+            }
+}
+
+template <typename T, int EPI>
+__device__ inline void epilogue(T *__restrict__ out, int64_t n, const T acc[3], const T *__restrict__ r,
+                                const T *__restrict__ diag, const T *__restrict__ xold, T omega) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T v = acc[c];
+        if (EPI == EPI_RESID) v = r[3 * n + c] - v;
+        if (EPI == EPI_JACOBI) v = xold[3 * n + c] + (omega * (r[3 * n + c] - v)) / diag[3 * n + c];
+        out[3 * n + c] = v;
+    }
+}
+
+// Finest level, standard voxels: y = sum_e E_e * K0 * u_e with fixed DOFs as identity rows and
+// zero columns (the BC-modified element matrices of the reference, TOP_GPU.m:704-720, 1729-1741).
+// K0 is column-major 24x24; its indices are compile-time constants after unrolling (scalar loads).
+template <typename T, int EPI>
+__global__ __launch_bounds__(256) void k_kfine(const T *__restrict__ u, T *__restrict__ out,
+                                               const T *__restrict__ E, const uint8_t *__restrict__ mask,
+                                               const T *__restrict__ K0, const Grid g, const T *__restrict__ r,
+                                               const T *__restrict__ diag, T omega) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodes()) return;
+    const int iy = (int)(n % g.NY);
+    const int64_t t = n / g.NY;
+    const int ix = (int)(t % g.NX), iz = (int)(t / g.NX);
+    const uint8_t mk = mask[n];
+    if (!(mk & NODE_ACTIVE)) {
+        out[3 * n] = 0;
+        out[3 * n + 1] = 0;
+        out[3 * n + 2] = 0;
+        return;
+    }
+    T un[27][3];
+    load_neighbours<T, true>(un, u, mask, g, iy, ix, iz, n);
+    T acc[3] = {0, 0, 0};
+#pragma unroll
+    for (int oz = -1; oz <= 0; ++oz)
+#pragma unroll
+        for (int ox = -1; ox <= 0; ++ox)
+#pragma unroll
+            for (int oy = -1; oy <= 0; ++oy) {
+                const int ey = iy + oy, ex = ix + ox, ez = iz + oz;
+                const bool ok = (unsigned)ey < (unsigned)g.EY && (unsigned)ex < (unsigned)g.EX &&
+                                (unsigned)ez < (unsigned)g.EZ;
+                const T Ee = ok ? E[ey + (int64_t)g.EY * (ex + (int64_t)g.EX * ez)] : (T)0;
+                if (Ee != (T)0) {
+                    const int a = vtx_of(-oy, -ox, -oz);
+                    T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const int k = nb_of(oy + vtx_y(b), ox + vtx_x(b), oz + vtx_z(b));
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            const T uv = un[k][d];
+                            s0 += uv * K0[(3 * b + d) + 24 * (3 * a + 0)];
+                            s1 += uv * K0[(3 * b + d) + 24 * (3 * a + 1)];
+                            s2 += uv * K0[(3 * b + d) + 24 * (3 * a + 2)];
+                        }
+                    }
+                    acc[0] += s0 * Ee;
+                    acc[1] += s1 * Ee;
+                    acc[2] += s2 * Ee;
+                }
+            }
+    if (mk & 1) acc[0] = u[3 * n];
+    if (mk & 2) acc[1] = u[3 * n + 1];
+    if (mk & 4) acc[2] = u[3 * n + 2];
+    epilogue<T, EPI>(out, n, acc, r, diag, u, omega);
+}
+
+// Per-element matrices in entry-major ("SoA") layout: Km[i * ldk + e], i = r + 24c.
+// Coarse levels (Galerkin Ks, MASKED = false) and the super-element finest level
+// (K_mat = I_mat * E_sub with BC masks, MASKED = true).  y_e = u_e^T * K_e (TOP_GPU.m:757-760).
+template <typename T, int EPI, bool MASKED>
+__global__ __launch_bounds__(256) void k_kelem(const T *__restrict__ u, T *__restrict__ out,
+                                               const T *__restrict__ Km, int64_t ldk,
+                                               const uint8_t *__restrict__ eleMask,
+                                               const uint8_t *__restrict__ mask, const Grid g,
+                                               const T *__restrict__ r, const T *__restrict__ diag, T omega) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodes()) return;
+    const int iy = (int)(n % g.NY);
+    const int64_t t = n / g.NY;
+    const int ix = (int)(t % g.NX), iz = (int)(t / g.NX);
+    const uint8_t mk = mask[n];
+    if (!(mk & NODE_ACTIVE)) {
+        out[3 * n] = 0;
+        out[3 * n + 1] = 0;
+        out[3 * n + 2] = 0;
+        return;
+    }
+    T un[27][3];
+    load_neighbours<T, MASKED>(un, u, mask, g, iy, ix, iz, n);
+    T acc[3] = {0, 0, 0};
+#pragma unroll
+    for (int oz = -1; oz <= 0; ++oz)
+#pragma unroll
+        for (int ox = -1; ox <= 0; ++ox)
+#pragma unroll
+            for (int oy = -1; oy <= 0; ++oy) {
+                const int ey = iy + oy, ex = ix + ox, ez = iz + oz;
+                const bool ok = (unsigned)ey < (unsigned)g.EY && (unsigned)ex < (unsigned)g.EX &&
+                                (unsigned)ez < (unsigned)g.EZ;
+                const int64_t e = ok ? ey + (int64_t)g.EY * (ex + (int64_t)g.EX * ez) : 0;
+                if (ok && eleMask[e]) {
+                    const int a = vtx_of(-oy, -ox, -oz);
+                    T s0 = 0, s1 = 0, s2 = 0;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const int k = nb_of(oy + vtx_y(b), ox + vtx_x(b), oz + vtx_z(b));
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) {
+                            const T uv = un[k][d];
+                            const int row = 3 * b + d;
+                            s0 += uv * Km[(int64_t)(row + 24 * (3 * a + 0)) * ldk + e];
+                            s1 += uv * Km[(int64_t)(row + 24 * (3 * a + 1)) * ldk + e];
+                            s2 += uv * Km[(int64_t)(row + 24 * (3 * a + 2)) * ldk + e];
+                        }
+                    }
+                    acc[0] += s0;
+                    acc[1] += s1;
+                    acc[2] += s2;
+                }
+            }
+    if (MASKED) {
+        if (mk & 1) acc[0] = u[3 * n];
+        if (mk & 2) acc[1] = u[3 * n + 1];
+        if (mk & 4) acc[2] = u[3 * n + 2];
+    }
+    epilogue<T, EPI>(out, n, acc, r, diag, u, omega);
+}
+
+// Trilinear weight of coarse vertex j at fine offset (a, b, c) of a span-S block; exact in binary.
+template <typename T, int S>
+__device__ inline T ri_w(int a, int b, int c, int j) {
+    const T ty = (T)a / S, tx = (T)b / S, tz = (T)c / S;
+    const T wy = vtx_y(j) ? ty : (T)1 - ty;
+    const T wx = vtx_x(j) ? tx : (T)1 - tx;
+    const T wz = vtx_z(j) ? tz : (T)1 - tz;
+    return wy * wx * wz;
+}
+
+// Restriction r_c = R r_f (Solving_RestrictResidual, TOP_GPU.m:817-847): for every coarse node J,
+// sum over the non-empty coarse elements around J of RI(:,J)' * (r_f ./ coeff) on the element's
+// (S+1)^3 fine nodes.  coeff holds transferMatCoeffi per fine grid node (1 where unused).
+template <typename T, int S>
+__global__ __launch_bounds__(256) void k_restrict(const T *__restrict__ rf, T *__restrict__ rc,
+                                                  const T *__restrict__ coeff, const uint8_t *__restrict__ cMask,
+                                                  const uint8_t *__restrict__ cEle, const Grid gc, const Grid gf) {
+    const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (J >= gc.nodes()) return;
+    const int jy = (int)(J % gc.NY);
+    const int64_t t = J / gc.NY;
+    const int jx = (int)(t % gc.NX), jz = (int)(t / gc.NX);
+    T a0 = 0, a1 = 0, a2 = 0;
+    if (cMask[J] & NODE_ACTIVE) {
+        for (int oz = -1; oz <= 0; ++oz)
+            for (int ox = -1; ox <= 0; ++ox)
+                for (int oy = -1; oy <= 0; ++oy) {
+                    const int ey = jy + oy, ex = jx + ox, ez = jz + oz;
+                    if ((unsigned)ey >= (unsigned)gc.EY || (unsigned)ex >= (unsigned)gc.EX ||
+                        (unsigned)ez >= (unsigned)gc.EZ)
+                        continue;
+                    if (!cEle[ey + (int64_t)gc.EY * (ex + (int64_t)gc.EX * ez)]) continue;
+                    const int jv = vtx_of(-oy, -ox, -oz);
+                    for (int c = 0; c <= S; ++c)
+                        for (int b = 0; b <= S; ++b)
+#pragma unroll
+                            for (int a = 0; a <= S; ++a) {
+                                const T w = ri_w<T, S>(a, b, c, jv);
+                                if (w == (T)0) continue;
+                                const int64_t i = (int64_t)(S * ey + a) +
+                                                  (int64_t)gf.NY * ((S * ex + b) + (int64_t)gf.NX * (S * ez + c));
+                                const T cf = coeff[i];
+                                a0 += w * (rf[3 * i] / cf);
+                                a1 += w * (rf[3 * i + 1] / cf);
+                                a2 += w * (rf[3 * i + 2] / cf);
+                            }
+                }
+    }
+    rc[3 * J] = a0;
+    rc[3 * J + 1] = a1;
+    rc[3 * J + 2] = a2;
+}
+
+// Prolongation x_f (+)= P x_c (Solving_InterpolationDeviation, TOP_GPU.m:848-878): every mesh
+// node of the fine grid averages the trilinear interpolants of the non-empty coarse elements
+// that contain it (sum, then divide by transferMatCoeffi).
+template <typename T, int S, bool ADD>
+__global__ __launch_bounds__(256) void k_prolong(const T *__restrict__ xc, T *__restrict__ xf,
+                                                 const T *__restrict__ coeff, const uint8_t *__restrict__ fMask,
+                                                 const uint8_t *__restrict__ cEle, const Grid gc, const Grid gf) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= gf.nodes()) return;
+    const int iy = (int)(i % gf.NY);
+    const int64_t t = i / gf.NY;
+    const int ix = (int)(t % gf.NX), iz = (int)(t / gf.NX);
+    T a0 = 0, a1 = 0, a2 = 0;
+    const bool act = fMask[i] & NODE_ACTIVE;
+    if (act) {
+        const int qy = iy / S, qx = ix / S, qz = iz / S;
+        const int ny0 = (iy % S == 0) ? qy - 1 : qy, nx0 = (ix % S == 0) ? qx - 1 : qx, nz0 = (iz % S == 0) ? qz - 1 : qz;
+        for (int ez = nz0; ez <= qz; ++ez)
+            for (int ex = nx0; ex <= qx; ++ex)
+                for (int ey = ny0; ey <= qy; ++ey) {
+                    if ((unsigned)ey >= (unsigned)gc.EY || (unsigned)ex >= (unsigned)gc.EX ||
+                        (unsigned)ez >= (unsigned)gc.EZ)
+                        continue;
+                    if (!cEle[ey + (int64_t)gc.EY * (ex + (int64_t)gc.EX * ez)]) continue;
+                    const int a = iy - S * ey, b = ix - S * ex, c = iz - S * ez;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int64_t Jn = (int64_t)(ey + vtx_y(j)) +
+                                           (int64_t)gc.NY * ((ex + vtx_x(j)) + (int64_t)gc.NX * (ez + vtx_z(j)));
+                        const T w = ri_w<T, S>(a, b, c, j);
+                        a0 += w * xc[3 * Jn];
+                        a1 += w * xc[3 * Jn + 1];
+                        a2 += w * xc[3 * Jn + 2];
+                    }
+                }
+        const T cf = coeff[i];
+        a0 = a0 / cf;
+        a1 = a1 / cf;
+        a2 = a2 / cf;
+    }
+    if (ADD) {
+        xf[3 * i] += a0;
+        xf[3 * i + 1] += a1;
+        xf[3 * i + 2] += a2;
+    } else {
+        xf[3 * i] = a0;
+        xf[3 * i + 1] = a1;
+        xf[3 * i + 2] = a2;
+    }
+}
+
+// x = w * (r ./ d)  (pre-smoothing, TOP_GPU.m:784)
+template <typename T>
+__global__ void k_jacobi0(const T *__restrict__ r, const T *__restrict__ d, T *__restrict__ x, T w, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = w * (r[i] / d[i]);
+}
+
+// out = x + (w*r) ./ d  ('Adapted' post-smoothing, TOP_GPU.m:807-809)
+template <typename T>
+__global__ void k_jacobi_adapted(const T *__restrict__ x, const T *__restrict__ r, const T *__restrict__ d,
+                                 T *__restrict__ out, T w, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = x[i] + (w * r[i]) / d[i];
+}
+
+// ---- conversions / packing ----
+template <typename A, typename B>
+__global__ void k_convert(const A *__restrict__ a, B *__restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (B)a[i];
+}
+
+// compressed interleaved (reference order) -> full grid; nodFull0 = 0-based full index per solid node
+template <typename A, typename B>
+__global__ void k_unpack(const A *__restrict__ c, B *__restrict__ f, const int32_t *__restrict__ nodFull0, int64_t Nn) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Nn) return;
+    const int64_t n = nodFull0[k];
+    f[3 * n] = (B)c[3 * k];
+    f[3 * n + 1] = (B)c[3 * k + 1];
+    f[3 * n + 2] = (B)c[3 * k + 2];
+}
+
+template <typename A, typename B>
+__global__ void k_pack(const A *__restrict__ f, B *__restrict__ c, const int32_t *__restrict__ nodFull0, int64_t Nn) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= Nn) return;
+    const int64_t n = nodFull0[k];
+    c[3 * k] = (B)f[3 * n];
+    c[3 * k + 1] = (B)f[3 * n + 1];
+    c[3 * k + 2] = (B)f[3 * n + 2];
+}
+
+// ---- deterministic reductions (fixed grid, fixed order) ----
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+
+template <typename T>
+__device__ inline double block_sum(double v) {
+    __shared__ double sh[RED_THREADS];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// partial[b] = sum over this block's fixed contiguous chunk of a[i]*b[i]
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_dot_partial(const T *__restrict__ a, const T *__restrict__ b,
+                                                             int64_t n, double *__restrict__ partial) {
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s += (double)a[i] * (double)b[i];
+    s = block_sum<T>(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[slot] = sum(partial[0..nb))
+static __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__restrict__ partial, int nb,
+                                                              double *__restrict__ out, int slot) {
+    double s = 0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[i];
+    s = block_sum<double>(s);
+    if (threadIdx.x == 0) out[slot] = s;
+}
+
+}  // namespace tg

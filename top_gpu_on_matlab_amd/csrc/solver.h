@@ -1,0 +1,68 @@
+// Device solver state (see solver.hip).
+#pragma once
+
+#include "coarse.h"
+#include "kernels.h"
+#include "model.h"
+
+namespace tg {
+
+struct DevLevel {
+    Grid g{};
+    int span = 0;
+    int64_t Nn = 0, Ne = 0, ldk = 0;       // ldk: entry stride of element matrices (padded element count)
+    DevBuf<uint8_t> nodeMask, eleMask;
+    DevBuf<int32_t> nodFull0, eleFull0;    // solid node / element -> 0-based full-grid index
+    DevBuf<double> diag64, r64, x64, t64;  // Jacobi diagonal and V-cycle vectors (full grid, interleaved)
+    DevBuf<float> diag32, r32, x32, t32;
+    DevBuf<double> Ks64;                   // element matrices, entry-major [576 x ldk]
+    DevBuf<float> Ks32;
+    DevBuf<double> coeffF64;               // transferMatCoeffi on the finer grid's nodes (levels >= 2)
+    DevBuf<float> coeffF32;
+    DevBuf<double> P;                      // Pch / Psub [24 x 24 x span^3] (levels >= 2)
+};
+
+void build_pch(int span, std::vector<double> &P);
+
+struct Solver {
+    const tg_model &model;
+    hipStream_t st;
+    int L = 0, prec = 0;
+    double omega = 0.6;
+    bool adapted = false, super = false;
+    bool has_modulus = false, assembled = false;
+    std::vector<DevLevel> lv;
+    CoarseSolver coarse;
+    DevBuf<double> K0_64, E64, Esub, Imat, uFree, uFixed;
+    DevBuf<float> K0_32, E32;
+    DevBuf<int32_t> fullUid;
+    DevBuf<double> b, y, r, z, p, v, partial, scal;
+    DevBuf<float> rin32, zout32;
+    double *h_scal = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0;
+    int last_its = 0;
+
+    Solver(const tg_model &m, hipStream_t s);
+    ~Solver();
+    void set_modulus(const double *dE);
+    void assemble();
+    template <typename T>
+    void kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T *dg);
+    template <typename T>
+    void restrict_level(int c, const T *rf, T *rc);
+    template <typename T>
+    void prolong_level(int c, const T *xc, T *xf, bool add);
+    template <typename T>
+    void vcycle_t(const T *r0, T *z0);
+    void vcycle(const double *r0, double *z0);
+    double dot(const double *a, const double *b, int slot);
+    double read_scalar(int slot);
+    void pcg(const double *b_c, double *y_c, double tol, int maxIT, int warm, int *its, double *res);
+};
+
+}  // namespace tg
+
+struct tg_solver {
+    tg::Solver *s = nullptr;
+};

@@ -1,0 +1,807 @@
+// Device-resident multigrid-preconditioned CG for the TOP_GPU hot path.
+//
+// Per level the solver keeps structured full-grid data (kernels.h): node masks, Jacobi
+// diagonals, element matrices (entry-major), transfer coefficients and V-cycle vectors.
+// FP64 mode mirrors the reference's double path.  Mixed mode runs the V-cycle in FP32
+// (coarsest solve in FP64) inside an FP64 PCG whose operator, dots and updates stay FP64.
+#include <cmath>
+
+#include "coarse.h"
+#include "galerkin.h"
+#include "kernels.h"
+#include "model.h"
+#include "solver.h"
+
+namespace tg {
+namespace {
+
+// ---- Galerkin sources, structured layouts -------------------------------------------------
+struct ChildGrid {
+    int span, cEY, cEX, fEY, fEX;
+    const uint8_t *fineSolid;
+    int64_t ce;
+    __device__ int child(int s) const {
+        const int a = s % span, b = (s / span) % span, c = s / (span * span);
+        const int cy = (int)(ce % cEY), cx = (int)((ce / cEY) % cEX), cz = (int)(ce / ((int64_t)cEY * cEX));
+        const int64_t f = (int64_t)(span * cy + a) + (int64_t)fEY * ((span * cx + b) + (int64_t)fEX * (span * cz + c));
+        return fineSolid[f] ? (int)f : -1;
+    }
+};
+
+struct SrcDevL2 : ChildGrid {
+    const double *E, *K0, *uf, *ux;
+    const int32_t *uid;
+    __device__ double kch(int f, int i) const {
+        const int32_t u = uid[f];
+        if (u > 0) return uf[(size_t)(u - 1) * 576 + i] * E[f] + ux[(size_t)(u - 1) * 576 + i];
+        return K0[i] * E[f];
+    }
+};
+
+struct SrcDevHigher : ChildGrid {
+    const double *prev;
+    int64_t ldp;
+    __device__ double kch(int f, int i) const { return prev[(int64_t)i * ldp + f]; }
+};
+
+struct SrcDevSuper : ChildGrid {
+    const double *Kmat, *Esub, *uf, *ux;
+    int64_t ldk;
+    const int32_t *uid;
+    __device__ double kch(int f, int i) const {
+        const int32_t u = uid[f];
+        if (u > 0) {
+            const size_t fb = (size_t)(u - 1) * 576 * 64, mb = (size_t)f * 64;
+            double acc = 0;
+            for (int q = 0; q < 64; ++q) acc += uf[fb + (size_t)q * 576 + i] * Esub[mb + q];
+            return acc + ux[(size_t)(u - 1) * 576 + i];
+        }
+        return Kmat[(int64_t)i * ldk + f];
+    }
+};
+
+template <class Src>
+__global__ __launch_bounds__(GK_THREADS) void k_galerkin_dev(Src src, const uint8_t *__restrict__ cSolid,
+                                                             int nCh, const double *__restrict__ P,
+                                                             double *__restrict__ Ks, int64_t ldk) {
+    __shared__ double sB[576], sK[576], sT[576];
+    const int64_t ce = blockIdx.x;
+    if (!cSolid[ce]) return;
+    src.ce = ce;
+    double acc[9];
+    galerkin_block(acc, src, nCh, P, sB, sK, sT);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ks[(int64_t)galerkin_entry(i) * ldk + ce] = acc[i];
+}
+
+// K_mat = I_mat * E_sub per element (TOP_GPU.m:1101-1103), entry-major output.
+__global__ void k_kmat(const double *__restrict__ Imat, const double *__restrict__ Esub, double *__restrict__ Km,
+                       int64_t ldk, int64_t nE) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nE * 576) return;
+    const int64_t e = t % nE, i = t / nE;
+    double acc = 0;
+    for (int q = 0; q < 64; ++q) acc += Imat[i + 576 * q] * Esub[q + 64 * e];
+    Km[i * ldk + e] = acc;
+}
+
+// Jacobi diagonal: sum over solid elements around the node of the element matrix diagonal.
+// MODE 0: E_e * K0(j,j) (finest, standard); 1: Km entry-major (super-element finest / coarse).
+// Fixed DOFs (finest level) get the identity row's 1; nodes outside the mesh get 1.
+template <int MODE>
+__global__ void k_diag(double *__restrict__ diag, const double *__restrict__ E, const double *__restrict__ K0,
+                       const double *__restrict__ Km, int64_t ldk, const uint8_t *__restrict__ eleMask,
+                       const uint8_t *__restrict__ mask, const Grid g) {
+    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= g.nodes()) return;
+    const uint8_t mk = mask[n];
+    if (!(mk & NODE_ACTIVE)) {
+        diag[3 * n] = diag[3 * n + 1] = diag[3 * n + 2] = 1.0;
+        return;
+    }
+    const int iy = (int)(n % g.NY);
+    const int64_t t = n / g.NY;
+    const int ix = (int)(t % g.NX), iz = (int)(t / g.NX);
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int oz = -1; oz <= 0; ++oz)
+        for (int ox = -1; ox <= 0; ++ox)
+            for (int oy = -1; oy <= 0; ++oy) {
+                const int ey = iy + oy, ex = ix + ox, ez = iz + oz;
+                if ((unsigned)ey >= (unsigned)g.EY || (unsigned)ex >= (unsigned)g.EX || (unsigned)ez >= (unsigned)g.EZ)
+                    continue;
+                const int64_t e = ey + (int64_t)g.EY * (ex + (int64_t)g.EX * ez);
+                if (!eleMask[e]) continue;
+                const int a = vtx_of(-oy, -ox, -oz);
+                if (MODE == 0) {
+                    a0 += K0[(3 * a + 0) * 25] * E[e];
+                    a1 += K0[(3 * a + 1) * 25] * E[e];
+                    a2 += K0[(3 * a + 2) * 25] * E[e];
+                } else {
+                    a0 += Km[(int64_t)((3 * a + 0) * 25) * ldk + e];
+                    a1 += Km[(int64_t)((3 * a + 1) * 25) * ldk + e];
+                    a2 += Km[(int64_t)((3 * a + 2) * 25) * ldk + e];
+                }
+            }
+    if (mk & 1) a0 = 1.0;
+    if (mk & 2) a1 = 1.0;
+    if (mk & 4) a2 = 1.0;
+    diag[3 * n] = a0;
+    diag[3 * n + 1] = a1;
+    diag[3 * n + 2] = a2;
+}
+
+__global__ void k_scatter_modulus(const double *__restrict__ Ec, double *__restrict__ Ef,
+                                  const int32_t *__restrict__ eleFull0, int64_t Ne, int per) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Ne * per) return;
+    const int64_t k = t / per, q = t % per;
+    Ef[(int64_t)eleFull0[k] * per + q] = Ec[t];
+}
+
+// y += lambda p; r -= lambda v; partial sums of r.r  (TOP_GPU.m:630-632, 637)
+__global__ __launch_bounds__(RED_THREADS) void k_pcg_update(double *__restrict__ y, double *__restrict__ r,
+                                                            const double *__restrict__ p, const double *__restrict__ v,
+                                                            int64_t n, const double *__restrict__ scal, int ix1, int ipv,
+                                                            double *__restrict__ partial) {
+    const double lam = scal[ix1] / scal[ipv];
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        y[i] = y[i] + lam * p[i];
+        const double rr = r[i] - lam * v[i];
+        r[i] = rr;
+        s += rr * rr;
+    }
+    s = block_sum<double>(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// p = z + (x2 / x1) p  (TOP_GPU.m:652)
+__global__ void k_pcg_direction(double *__restrict__ p, const double *__restrict__ z, int64_t n,
+                                const double *__restrict__ scal, int ix2, int ix1) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + scal[ix2] / scal[ix1] * p[i];
+}
+
+enum { S_PV = 0, S_RR = 1, S_BB = 2, S_XA = 3, S_XB = 4 };
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+void build_pch(int span, std::vector<double> &P) {
+    const int nb = span + 1, s3 = span * span * span;
+    // child order of Solving_SubEleNodMat (TOP_GPU.m:1938-2017): span 4 column-major, span 2 per
+    // z-layer in hex-vertex order (0,0),(1,0),(1,1),(0,1)
+    std::vector<int> cy, cx, cz;
+    for (int ez = 0; ez < span; ++ez) {
+        if (span == 2) {
+            const int oy[4] = {0, 1, 1, 0}, ox[4] = {0, 0, 1, 1};
+            for (int k = 0; k < 4; ++k) { cy.push_back(oy[k]); cx.push_back(ox[k]); cz.push_back(ez); }
+        } else {
+            for (int ex = 0; ex < span; ++ex)
+                for (int ey = 0; ey < span; ++ey) { cy.push_back(ey); cx.push_back(ex); cz.push_back(ez); }
+        }
+    }
+    P.assign((size_t)576 * s3, 0.0);
+    for (int s = 0; s < s3; ++s)
+        for (int v = 0; v < 8; ++v) {
+            const int a = cy[s] + VY[v], b = cx[s] + VX[v], c = cz[s] + VZ[v];
+            (void)nb;
+            for (int j = 0; j < 8; ++j) {
+                const double w = ri_weight(span, a, b, c, j);
+                for (int d = 0; d < 3; ++d) P[(size_t)576 * s + (3 * v + d) + 24 * (3 * j + d)] = w;
+            }
+        }
+}
+
+static Grid grid_of(const HostLevel &L) {
+    Grid g;
+    g.EY = L.resY;
+    g.EX = L.resX;
+    g.EZ = L.resZ;
+    g.NY = L.resY + 1;
+    g.NX = L.resX + 1;
+    g.NZ = L.resZ + 1;
+    return g;
+}
+
+Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
+    const char *id = "tg_solver_create:Input";
+    if (!m.finalized) fail(TG_ERR_STATE, id, "model has no boundary conditions (call tg_model_finalize first)");
+    L = m.numLevels;
+    prec = m.prm.precision;
+    omega = m.prm.weight_factor_jacobi;
+    adapted = m.prm.vcycle_type == 1;
+    super = m.prm.super_element != 0;
+    lv.resize(L);
+    for (int i = 0; i < L; ++i) {
+        const HostLevel &H = m.lv[i];
+        DevLevel &D = lv[i];
+        D.g = grid_of(H);
+        D.span = H.span;
+        D.Nn = H.Nn;
+        D.Ne = H.Ne;
+        D.ldk = ((D.g.elems() + 63) / 64) * 64;
+        std::vector<uint8_t> mask(D.g.nodes(), 0);
+        std::vector<int32_t> nf(H.Nn), ef(H.Ne);
+        for (int64_t k = 0; k < H.Nn; ++k) {
+            nf[k] = H.nodFull[k] - 1;
+            mask[nf[k]] = NODE_ACTIVE | (i == 0 ? (m.nodeFixMask[k] & 7) : 0);
+        }
+        for (int64_t k = 0; k < H.Ne; ++k) ef[k] = H.eleMapBack[k] - 1;
+        D.nodeMask.upload(mask, st);
+        D.eleMask.upload(H.solid, st);
+        D.nodFull0.upload(nf, st);
+        D.eleFull0.upload(ef, st);
+        const int64_t nv = 3 * D.g.nodes();
+        D.diag64.alloc(nv);
+        D.r64.alloc(nv);
+        D.x64.alloc(nv);
+        D.t64.alloc(nv);
+        if (prec == TG_PREC_MIXED) {
+            D.diag32.alloc(nv);
+            D.r32.alloc(nv);
+            D.x32.alloc(nv);
+            D.t32.alloc(nv);
+        }
+        if (i >= 1) {
+            D.Ks64.alloc(576 * D.ldk);
+            D.Ks64.zero(st);
+            if (prec == TG_PREC_MIXED) D.Ks32.alloc(576 * D.ldk);
+            // transfer coefficients on the finer grid (1 where a node is unused)
+            const HostLevel &F = m.lv[i - 1];
+            std::vector<double> cf(F.nodesFull(), 1.0);
+            for (int64_t n = 0; n < F.nodesFull(); ++n)
+                if (H.fineInterId[n]) cf[n] = H.coeff[H.fineInterId[n] - 1];
+            D.coeffF64.upload(cf, st);
+            if (prec == TG_PREC_MIXED) {
+                std::vector<float> c32(cf.begin(), cf.end());
+                D.coeffF32.upload(c32, st);
+            }
+            std::vector<double> P;
+            build_pch(H.span, P);
+            D.P.upload(P, st);
+        }
+    }
+    DevLevel &F = lv[0];
+    K0_64.upload(m.Ke, st);
+    if (prec == TG_PREC_MIXED) {
+        std::vector<float> k32(m.Ke.begin(), m.Ke.end());
+        K0_32.upload(k32, st);
+    }
+    const int64_t nEf = F.g.elems();
+    std::vector<int32_t> uid(nEf, 0);
+    for (int64_t k = 0; k < F.Ne; ++k) uid[m.lv[0].eleMapBack[k] - 1] = m.mapUniqueKes[k];
+    fullUid.upload(uid, st);
+    uFree.upload(m.uniqueFree, st);
+    uFixed.upload(m.uniqueFixed, st);
+    if (super) {
+        Imat.upload(m.Imat, st);
+        Esub.alloc(64 * nEf);
+        Esub.zero(st);
+        F.Ks64.alloc(576 * F.ldk);  // K_mat on the finest level
+        F.Ks64.zero(st);
+        if (prec == TG_PREC_MIXED) F.Ks32.alloc(576 * F.ldk);
+    } else {
+        E64.alloc(nEf);
+        E64.zero(st);
+        if (prec == TG_PREC_MIXED) E32.alloc(nEf);
+    }
+    const int64_t nv = 3 * F.g.nodes();
+    b.alloc(nv);
+    y.alloc(nv);
+    r.alloc(nv);
+    z.alloc(nv);
+    p.alloc(nv);
+    v.alloc(nv);
+    if (prec == TG_PREC_MIXED) {
+        rin32.alloc(nv);
+        zout32.alloc(nv);
+    }
+    partial.alloc(RED_BLOCKS);
+    scal.alloc(16);
+    scal.zero(st);
+    TG_HIP(hipHostMalloc((void **)&h_scal, 16 * sizeof(double), hipHostMallocDefault), id);
+    TG_HIP(hipEventCreate(&ev0), id);
+    TG_HIP(hipEventCreate(&ev1), id);
+    coarse.setup(m.lv[L - 1], lv[L - 1].g, st);
+    TG_HIP(hipStreamSynchronize(st), id);
+}
+
+Solver::~Solver() {
+    if (h_scal) (void)hipHostFree(h_scal);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+}
+
+void Solver::set_modulus(const double *dE) {
+    const char *id = "tg_solver_set_modulus:Input";
+    DevLevel &F = lv[0];
+    if (super) {
+        Esub.zero(st);
+        k_scatter_modulus<<<grid_for(F.Ne * 64, 256), 256, 0, st>>>(dE, Esub, F.eleFull0, F.Ne, 64);
+    } else {
+        E64.zero(st);
+        k_scatter_modulus<<<grid_for(F.Ne, 256), 256, 0, st>>>(dE, E64, F.eleFull0, F.Ne, 1);
+    }
+    TG_LAUNCH_CHECK(id);
+    has_modulus = true;
+    assembled = false;
+}
+
+void Solver::assemble() {
+    const char *id = "tg_solver_assemble:State";
+    if (!has_modulus) fail(TG_ERR_STATE, id, "set the element moduli first");
+    DevLevel &F = lv[0];
+    if (super) {
+        k_kmat<<<grid_for(F.g.elems() * 576, 256), 256, 0, st>>>(Imat, Esub, F.Ks64, F.ldk, F.g.elems());
+        TG_LAUNCH_CHECK(id);
+    }
+    for (int i = 1; i < L; ++i) {
+        DevLevel &C = lv[i], &Fi = lv[i - 1];
+        const int nCh = C.span * C.span * C.span;
+        ChildGrid cg{C.span, C.g.EY, C.g.EX, Fi.g.EY, Fi.g.EX, Fi.eleMask.p, 0};
+        const unsigned grid = (unsigned)C.g.elems();
+        if (i == 1 && !super) {
+            SrcDevL2 src;
+            static_cast<ChildGrid &>(src) = cg;
+            src.E = E64; src.K0 = K0_64; src.uf = uFree; src.ux = uFixed; src.uid = fullUid;
+            k_galerkin_dev<<<grid, GK_THREADS, 0, st>>>(src, C.eleMask, nCh, C.P, C.Ks64, C.ldk);
+        } else if (i == 1) {
+            SrcDevSuper src;
+            static_cast<ChildGrid &>(src) = cg;
+            src.Kmat = F.Ks64; src.ldk = F.ldk; src.Esub = Esub; src.uf = uFree; src.ux = uFixed; src.uid = fullUid;
+            k_galerkin_dev<<<grid, GK_THREADS, 0, st>>>(src, C.eleMask, nCh, C.P, C.Ks64, C.ldk);
+        } else {
+            SrcDevHigher src;
+            static_cast<ChildGrid &>(src) = cg;
+            src.prev = Fi.Ks64; src.ldp = Fi.ldk;
+            k_galerkin_dev<<<grid, GK_THREADS, 0, st>>>(src, C.eleMask, nCh, C.P, C.Ks64, C.ldk);
+        }
+        TG_LAUNCH_CHECK(id);
+    }
+    // Jacobi diagonals on levels 1..L-1 (TOP_GPU.m:1149-1239)
+    for (int i = 0; i < L - 1; ++i) {
+        DevLevel &D = lv[i];
+        const unsigned gr = grid_for(D.g.nodes(), 256);
+        if (i == 0 && !super)
+            k_diag<0><<<gr, 256, 0, st>>>(D.diag64, E64, K0_64, nullptr, 0, D.eleMask, D.nodeMask, D.g);
+        else
+            k_diag<1><<<gr, 256, 0, st>>>(D.diag64, nullptr, nullptr, D.Ks64, D.ldk, D.eleMask, D.nodeMask, D.g);
+        TG_LAUNCH_CHECK(id);
+    }
+    coarse.factor(lv[L - 1].Ks64, lv[L - 1].ldk, lv[L - 1].eleMask, lv[L - 1].nodeMask, st);
+    if (prec == TG_PREC_MIXED) {
+        for (int i = 0; i < L; ++i) {
+            DevLevel &D = lv[i];
+            if (i < L - 1)
+                k_convert<double, float><<<grid_for(3 * D.g.nodes(), 256), 256, 0, st>>>(D.diag64, D.diag32, 3 * D.g.nodes());
+            if ((i >= 1 && i < L - 1) || (i == 0 && super))
+                k_convert<double, float><<<grid_for(576 * D.ldk, 256), 256, 0, st>>>(D.Ks64, D.Ks32, 576 * D.ldk);
+        }
+        if (!super) k_convert<double, float><<<grid_for(F.g.elems(), 256), 256, 0, st>>>(E64, E32, F.g.elems());
+        TG_LAUNCH_CHECK(id);
+    }
+    assembled = true;
+}
+
+template <typename T>
+void Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T *dg) {
+    DevLevel &D = lv[i];
+    const unsigned gr = grid_for(D.g.nodes(), 256);
+    const T w = (T)omega;
+    const bool f64 = sizeof(T) == 8;
+    if (i == 0 && !super) {
+        const T *E = f64 ? (const T *)E64.p : (const T *)E32.p;
+        const T *K0 = f64 ? (const T *)K0_64.p : (const T *)K0_32.p;
+        if (epi == EPI_KU) k_kfine<T, EPI_KU><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, D.g, rr, dg, w);
+        else if (epi == EPI_RESID) k_kfine<T, EPI_RESID><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, D.g, rr, dg, w);
+        else k_kfine<T, EPI_JACOBI><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, D.g, rr, dg, w);
+    } else {
+        const T *Km = f64 ? (const T *)D.Ks64.p : (const T *)D.Ks32.p;
+        if (i == 0) {
+            if (epi == EPI_KU) k_kelem<T, EPI_KU, true><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else if (epi == EPI_RESID) k_kelem<T, EPI_RESID, true><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else k_kelem<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+        } else {
+            if (epi == EPI_KU) k_kelem<T, EPI_KU, false><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else if (epi == EPI_RESID) k_kelem<T, EPI_RESID, false><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else k_kelem<T, EPI_JACOBI, false><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+        }
+    }
+    TG_LAUNCH_CHECK("tg_solver:kbyu");
+}
+
+template <typename T>
+void Solver::restrict_level(int c, const T *rf, T *rc) {
+    DevLevel &C = lv[c], &F = lv[c - 1];
+    const T *cf = sizeof(T) == 8 ? (const T *)C.coeffF64.p : (const T *)C.coeffF32.p;
+    const unsigned gr = grid_for(C.g.nodes(), 256);
+    if (C.span == 4) k_restrict<T, 4><<<gr, 256, 0, st>>>(rf, rc, cf, C.nodeMask, C.eleMask, C.g, F.g);
+    else k_restrict<T, 2><<<gr, 256, 0, st>>>(rf, rc, cf, C.nodeMask, C.eleMask, C.g, F.g);
+    TG_LAUNCH_CHECK("tg_solver:restrict");
+}
+
+template <typename T>
+void Solver::prolong_level(int c, const T *xc, T *xf, bool add) {
+    DevLevel &C = lv[c], &F = lv[c - 1];
+    const T *cf = sizeof(T) == 8 ? (const T *)C.coeffF64.p : (const T *)C.coeffF32.p;
+    const unsigned gr = grid_for(F.g.nodes(), 256);
+    if (C.span == 4) {
+        if (add) k_prolong<T, 4, true><<<gr, 256, 0, st>>>(xc, xf, cf, F.nodeMask, C.eleMask, C.g, F.g);
+        else k_prolong<T, 4, false><<<gr, 256, 0, st>>>(xc, xf, cf, F.nodeMask, C.eleMask, C.g, F.g);
+    } else {
+        if (add) k_prolong<T, 2, true><<<gr, 256, 0, st>>>(xc, xf, cf, F.nodeMask, C.eleMask, C.g, F.g);
+        else k_prolong<T, 2, false><<<gr, 256, 0, st>>>(xc, xf, cf, F.nodeMask, C.eleMask, C.g, F.g);
+    }
+    TG_LAUNCH_CHECK("tg_solver:prolong");
+}
+
+template <typename T>
+static T *vec(DevLevel &D, int which) {
+    if (sizeof(T) == 8) return (T *)(which == 0 ? D.r64.p : (which == 1 ? D.x64.p : D.t64.p));
+    return (T *)(which == 0 ? D.r32.p : (which == 1 ? D.x32.p : D.t32.p));
+}
+
+template <typename T>
+static const T *diag_of(DevLevel &D) {
+    return sizeof(T) == 8 ? (const T *)D.diag64.p : (const T *)D.diag32.p;
+}
+
+// Solving_Vcycle (TOP_GPU.m:766-816) on full-grid vectors of precision T.
+template <typename T>
+void Solver::vcycle_t(const T *r0, T *z0) {
+    const T w = (T)omega;
+    std::vector<const T *> rr(L);
+    std::vector<T *> xx(L);
+    rr[0] = r0;
+    for (int i = 1; i < L; ++i) rr[i] = vec<T>(lv[i], 0);
+    for (int i = 0; i < L; ++i) xx[i] = vec<T>(lv[i], 1);
+    for (int i = 0; i < L - 1; ++i) {
+        DevLevel &D = lv[i];
+        const int64_t nv = 3 * D.g.nodes();
+        k_jacobi0<T><<<grid_for(nv, 256), 256, 0, st>>>(rr[i], diag_of<T>(D), xx[i], w, nv);
+        if (adapted) {
+            restrict_level<T>(i + 1, rr[i], (T *)rr[i + 1]);
+        } else {
+            T *d = vec<T>(D, 2);
+            kbyu_level<T>(i, xx[i], d, EPI_RESID, rr[i], nullptr);
+            restrict_level<T>(i + 1, d, (T *)rr[i + 1]);
+        }
+    }
+    coarse.solve<T>(rr[L - 1], xx[L - 1], lv[L - 1].nodeMask, st);
+    for (int i = L - 2; i >= 0; --i) {
+        DevLevel &D = lv[i];
+        prolong_level<T>(i + 1, xx[i + 1], xx[i], true);
+        // the smoothed iterate goes to a second buffer (neighbours still read the old one);
+        // on the finest level straight into the caller's output
+        T *out = (i == 0) ? z0 : vec<T>(D, 2);
+        if (adapted) {
+            const int64_t nv = 3 * D.g.nodes();
+            k_jacobi_adapted<T><<<grid_for(nv, 256), 256, 0, st>>>(xx[i], rr[i], diag_of<T>(D), out, w, nv);
+        } else {
+            kbyu_level<T>(i, xx[i], out, EPI_JACOBI, rr[i], diag_of<T>(D));
+        }
+        xx[i] = out;
+    }
+    TG_LAUNCH_CHECK("tg_solver:vcycle");
+}
+
+void Solver::vcycle(const double *r0, double *z0) {
+    if (prec == TG_PREC_MIXED) {
+        DevLevel &F = lv[0];
+        const int64_t nv = 3 * F.g.nodes();
+        k_convert<double, float><<<grid_for(nv, 256), 256, 0, st>>>(r0, rin32, nv);
+        vcycle_t<float>(rin32, zout32);
+        k_convert<float, double><<<grid_for(nv, 256), 256, 0, st>>>(zout32, z0, nv);
+    } else {
+        vcycle_t<double>(r0, z0);
+    }
+}
+
+double Solver::dot(const double *a, const double *bb, int slot) {
+    const int64_t n = 3 * lv[0].g.nodes();
+    k_dot_partial<double><<<RED_BLOCKS, RED_THREADS, 0, st>>>(a, bb, n, partial);
+    k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, slot);
+    TG_LAUNCH_CHECK("tg_solver:dot");
+    return 0;
+}
+
+double Solver::read_scalar(int slot) {
+    TG_HIP(hipMemcpyAsync(h_scal + slot, scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, st), "tg_solver:pcg");
+    TG_HIP(hipStreamSynchronize(st), "tg_solver:pcg");
+    return h_scal[slot];
+}
+
+// Solving_PCG (TOP_GPU.m:602-667).
+void Solver::pcg(const double *b_c, double *y_c, double tol, int maxIT, int warm, int *its_out, double *res_out) {
+    const char *id = "tg_solver_pcg:State";
+    if (!assembled) fail(TG_ERR_STATE, id, "assemble before solving");
+    DevLevel &F = lv[0];
+    const int64_t nv = 3 * F.g.nodes();
+    TG_HIP(hipEventRecord(ev0, st), id);
+    b.zero(st);
+    k_unpack<double, double><<<grid_for(F.Nn, 256), 256, 0, st>>>(b_c, b, F.nodFull0, F.Nn);
+    y.zero(st);
+    if (warm) k_unpack<double, double><<<grid_for(F.Nn, 256), 256, 0, st>>>(y_c, y, F.nodFull0, F.Nn);
+    TG_LAUNCH_CHECK(id);
+    dot(b, b, S_BB);
+    const double normB = std::sqrt(read_scalar(S_BB));
+    if (warm) {
+        kbyu_level<double>(0, y, r, EPI_RESID, b, nullptr);
+    } else {
+        TG_HIP(hipMemcpyAsync(r.p, b.p, nv * sizeof(double), hipMemcpyDeviceToDevice, st), id);
+    }
+    vcycle(r, z);
+    TG_HIP(hipMemcpyAsync(p.p, z.p, nv * sizeof(double), hipMemcpyDeviceToDevice, st), id);
+    int sx1 = S_XA, sx2 = S_XB;
+    dot(z, r, sx1);
+    int its = 0;
+    double res = 0;
+    for (its = 1; its <= maxIT; ++its) {
+        kbyu_level<double>(0, p, v, EPI_KU, nullptr, nullptr);
+        dot(p, v, S_PV);
+        k_pcg_update<<<RED_BLOCKS, RED_THREADS, 0, st>>>(y, r, p, v, nv, scal, sx1, S_PV, partial);
+        k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR);
+        TG_LAUNCH_CHECK(id);
+        res = std::sqrt(read_scalar(S_RR)) / normB;
+        if (!std::isfinite(res)) fail(TG_ERR_NUMERIC, "tg_solver_pcg:Numeric", "non-finite residual at iteration %d", its);
+        if (res < tol) break;
+        vcycle(r, z);
+        dot(z, r, sx2);
+        k_pcg_direction<<<grid_for(nv, 256), 256, 0, st>>>(p, z, nv, scal, sx2, sx1);
+        TG_LAUNCH_CHECK(id);
+        std::swap(sx1, sx2);
+    }
+    if (its > maxIT) its = maxIT;
+    k_pack<double, double><<<grid_for(F.Nn, 256), 256, 0, st>>>(y, y_c, F.nodFull0, F.Nn);
+    TG_LAUNCH_CHECK(id);
+    TG_HIP(hipEventRecord(ev1, st), id);
+    TG_HIP(hipEventSynchronize(ev1), id);
+    float ms = 0;
+    TG_HIP(hipEventElapsedTime(&ms, ev0, ev1), id);
+    last_ms = ms;
+    last_its = its;
+    if (its_out) *its_out = its;
+    if (res_out) *res_out = res;
+}
+
+}  // namespace tg
+
+// ---------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------
+using namespace tg;
+
+namespace {
+
+Solver &S_of(tg_solver *s, const char *id) {
+    if (!s || !s->s) fail(TG_ERR_ARGS, id, "null solver");
+    return *s->s;
+}
+
+int level_index(const Solver &S, int32_t level, const char *id) {
+    if (level < 1 || level > S.L) fail(TG_ERR_ARGS, id, "level %d out of range 1..%d", level, S.L);
+    return level - 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_solver_create(const tg_model *m, void *stream, tg_solver **out) {
+    return guard("tg_solver_create:Input", [&] {
+        if (!m || !out) fail(TG_ERR_ARGS, "tg_solver_create:Input", "null argument");
+        auto *h = new tg_solver();
+        try {
+            h->s = new Solver(*m, (hipStream_t)stream);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+void tg_solver_destroy(tg_solver *s) {
+    if (!s) return;
+    if (s->s) (void)hipStreamSynchronize(s->s->st);
+    delete s->s;
+    delete s;
+}
+
+int tg_solver_set_modulus(tg_solver *s, const double *d_E) {
+    return guard("tg_solver_set_modulus:Input", [&] {
+        Solver &S = S_of(s, "tg_solver_set_modulus:Input");
+        if (!d_E) fail(TG_ERR_ARGS, "tg_solver_set_modulus:Input", "null modulus");
+        S.set_modulus(d_E);
+    });
+}
+
+int tg_solver_assemble(tg_solver *s) {
+    return guard("tg_solver_assemble:State", [&] { S_of(s, "tg_solver_assemble:State").assemble(); });
+}
+
+int tg_solver_kbyu(tg_solver *s, int32_t level, const double *d_u, double *d_y) {
+    return guard("tg_solver_kbyu:Input", [&] {
+        const char *id = "tg_solver_kbyu:Input";
+        Solver &S = S_of(s, id);
+        const int i = level_index(S, level, id);
+        if (!S.assembled) fail(TG_ERR_STATE, "tg_solver_kbyu:State", "assemble first");
+        if (!d_u || !d_y) fail(TG_ERR_ARGS, id, "null vector");
+        DevLevel &D = S.lv[i];
+        D.x64.zero(S.st);
+        k_unpack<double, double><<<grid_for(D.Nn, 256), 256, 0, S.st>>>(d_u, D.x64, D.nodFull0, D.Nn);
+        S.kbyu_level<double>(i, D.x64, D.t64, EPI_KU, nullptr, nullptr);
+        k_pack<double, double><<<grid_for(D.Nn, 256), 256, 0, S.st>>>(D.t64, d_y, D.nodFull0, D.Nn);
+        TG_LAUNCH_CHECK(id);
+        TG_HIP(hipStreamSynchronize(S.st), id);
+    });
+}
+
+int tg_solver_vcycle(tg_solver *s, const double *d_r, double *d_z) {
+    return guard("tg_solver_vcycle:Input", [&] {
+        const char *id = "tg_solver_vcycle:Input";
+        Solver &S = S_of(s, id);
+        if (!S.assembled) fail(TG_ERR_STATE, "tg_solver_vcycle:State", "assemble first");
+        DevLevel &F = S.lv[0];
+        S.r.zero(S.st);
+        k_unpack<double, double><<<grid_for(F.Nn, 256), 256, 0, S.st>>>(d_r, S.r, F.nodFull0, F.Nn);
+        S.vcycle(S.r, S.z);
+        k_pack<double, double><<<grid_for(F.Nn, 256), 256, 0, S.st>>>(S.z, d_z, F.nodFull0, F.Nn);
+        TG_LAUNCH_CHECK(id);
+        TG_HIP(hipStreamSynchronize(S.st), id);
+    });
+}
+
+int tg_solver_restrict(tg_solver *s, int32_t coarse_level, const double *d_rf, double *d_rc) {
+    return guard("tg_solver_restrict:Input", [&] {
+        const char *id = "tg_solver_restrict:Input";
+        Solver &S = S_of(s, id);
+        const int c = level_index(S, coarse_level, id);
+        if (c < 1) fail(TG_ERR_ARGS, id, "coarse_level must be >= 2");
+        DevLevel &C = S.lv[c], &F = S.lv[c - 1];
+        F.t64.zero(S.st);
+        k_unpack<double, double><<<grid_for(F.Nn, 256), 256, 0, S.st>>>(d_rf, F.t64, F.nodFull0, F.Nn);
+        S.restrict_level<double>(c, F.t64, C.r64);
+        k_pack<double, double><<<grid_for(C.Nn, 256), 256, 0, S.st>>>(C.r64, d_rc, C.nodFull0, C.Nn);
+        TG_LAUNCH_CHECK(id);
+        TG_HIP(hipStreamSynchronize(S.st), id);
+    });
+}
+
+int tg_solver_interpolate(tg_solver *s, int32_t coarse_level, const double *d_xc, double *d_xf) {
+    return guard("tg_solver_interpolate:Input", [&] {
+        const char *id = "tg_solver_interpolate:Input";
+        Solver &S = S_of(s, id);
+        const int c = level_index(S, coarse_level, id);
+        if (c < 1) fail(TG_ERR_ARGS, id, "coarse_level must be >= 2");
+        DevLevel &C = S.lv[c], &F = S.lv[c - 1];
+        C.x64.zero(S.st);
+        k_unpack<double, double><<<grid_for(C.Nn, 256), 256, 0, S.st>>>(d_xc, C.x64, C.nodFull0, C.Nn);
+        S.prolong_level<double>(c, C.x64, F.t64, false);
+        k_pack<double, double><<<grid_for(F.Nn, 256), 256, 0, S.st>>>(F.t64, d_xf, F.nodFull0, F.Nn);
+        TG_LAUNCH_CHECK(id);
+        TG_HIP(hipStreamSynchronize(S.st), id);
+    });
+}
+
+int tg_solver_coarse_solve(tg_solver *s, const double *d_r, double *d_x) {
+    return guard("tg_solver_coarse_solve:Input", [&] {
+        const char *id = "tg_solver_coarse_solve:Input";
+        Solver &S = S_of(s, id);
+        if (!S.assembled) fail(TG_ERR_STATE, "tg_solver_coarse_solve:State", "assemble first");
+        DevLevel &C = S.lv[S.L - 1];
+        C.r64.zero(S.st);
+        k_unpack<double, double><<<grid_for(C.Nn, 256), 256, 0, S.st>>>(d_r, C.r64, C.nodFull0, C.Nn);
+        S.coarse.solve<double>(C.r64, C.x64, C.nodeMask, S.st);
+        k_pack<double, double><<<grid_for(C.Nn, 256), 256, 0, S.st>>>(C.x64, d_x, C.nodFull0, C.Nn);
+        TG_LAUNCH_CHECK(id);
+        TG_HIP(hipStreamSynchronize(S.st), id);
+    });
+}
+
+int tg_solver_pcg(tg_solver *s, const double *d_b, double *d_y, double tol, int32_t max_it, int32_t warm,
+                  int32_t *its, double *resnorm) {
+    return guard("tg_solver_pcg:Input", [&] {
+        Solver &S = S_of(s, "tg_solver_pcg:Input");
+        if (!d_b || !d_y || max_it < 1) fail(TG_ERR_ARGS, "tg_solver_pcg:Input", "bad arguments");
+        int it = 0;
+        double res = 0;
+        S.pcg(d_b, d_y, tol, max_it, warm, &it, &res);
+        if (its) *its = it;
+        if (resnorm) *resnorm = res;
+    });
+}
+
+int tg_solver_get_diag(tg_solver *s, int32_t level, double *d_out) {
+    return guard("tg_solver_get_diag:Input", [&] {
+        const char *id = "tg_solver_get_diag:Input";
+        Solver &S = S_of(s, id);
+        const int i = level_index(S, level, id);
+        if (i >= S.L - 1) fail(TG_ERR_ARGS, id, "the coarsest level has no Jacobi diagonal");
+        if (!S.assembled) fail(TG_ERR_STATE, "tg_solver_get_diag:State", "assemble first");
+        DevLevel &D = S.lv[i];
+        k_pack<double, double><<<grid_for(D.Nn, 256), 256, 0, S.st>>>(D.diag64, d_out, D.nodFull0, D.Nn);
+        TG_LAUNCH_CHECK(id);
+        TG_HIP(hipStreamSynchronize(S.st), id);
+    });
+}
+
+}  // extern "C"
+
+namespace tg {
+namespace {
+// d_Ks(:,:,k) of the reference from the entry-major full-grid layout
+__global__ void k_ks_to_reference(const double *__restrict__ Km, int64_t ldk, const int32_t *__restrict__ eleFull0,
+                                  int64_t Ne, double *__restrict__ out) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Ne * 576) return;
+    const int64_t k = t / 576, i = t % 576;
+    out[t] = Km[i * ldk + eleFull0[k]];
+}
+}  // namespace
+}  // namespace tg
+
+extern "C" {
+
+int tg_solver_get_ks(tg_solver *s, int32_t level, double *d_out) {
+    return guard("tg_solver_get_ks:Input", [&] {
+        const char *id = "tg_solver_get_ks:Input";
+        Solver &S = S_of(s, id);
+        const int i = level_index(S, level, id);
+        if (i == 0 && !S.super) fail(TG_ERR_ARGS, id, "level 1 has no per-element matrices in standard mode");
+        if (!S.assembled) fail(TG_ERR_STATE, "tg_solver_get_ks:State", "assemble first");
+        DevLevel &D = S.lv[i];
+        k_ks_to_reference<<<grid_for(D.Ne * 576, 256), 256, 0, S.st>>>(D.Ks64, D.ldk, D.eleFull0, D.Ne, d_out);
+        TG_LAUNCH_CHECK(id);
+        TG_HIP(hipStreamSynchronize(S.st), id);
+    });
+}
+
+int tg_solver_last_timing(const tg_solver *s, double *ms_total) {
+    if (!s || !s->s || !ms_total) return TG_ERR_ARGS;
+    *ms_total = s->s->last_ms;
+    return TG_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Average duration of one K*u kernel launch at `level` (precision 0: FP64, 1: FP32 V-cycle copy),
+// timed with HIP events around `reps` back-to-back launches on the solver's stream.
+int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t reps, double *ms_per_launch) {
+    return guard("tg_solver_time_kbyu:Input", [&] {
+        const char *id = "tg_solver_time_kbyu:Input";
+        Solver &S = S_of(s, id);
+        const int i = level_index(S, level, id);
+        if (!S.assembled) fail(TG_ERR_STATE, "tg_solver_time_kbyu:State", "assemble first");
+        if (reps < 1 || !ms_per_launch) fail(TG_ERR_ARGS, id, "bad arguments");
+        if (precision == 1 && S.prec != TG_PREC_MIXED) fail(TG_ERR_ARGS, id, "FP32 operators exist in mixed mode only");
+        if (i == S.L - 1 && precision == 1) fail(TG_ERR_ARGS, id, "the coarsest level has no FP32 operator");
+        DevLevel &D = S.lv[i];
+        hipEvent_t a, b;
+        TG_HIP(hipEventCreate(&a), id);
+        TG_HIP(hipEventCreate(&b), id);
+        if (precision == 1) {
+            S.kbyu_level<float>(i, D.x32, D.t32, EPI_KU, nullptr, nullptr);
+            TG_HIP(hipEventRecord(a, S.st), id);
+            for (int k = 0; k < reps; ++k) S.kbyu_level<float>(i, D.x32, D.t32, EPI_KU, nullptr, nullptr);
+        } else {
+            S.kbyu_level<double>(i, D.x64, D.t64, EPI_KU, nullptr, nullptr);
+            TG_HIP(hipEventRecord(a, S.st), id);
+            for (int k = 0; k < reps; ++k) S.kbyu_level<double>(i, D.x64, D.t64, EPI_KU, nullptr, nullptr);
+        }
+        TG_HIP(hipEventRecord(b, S.st), id);
+        TG_HIP(hipEventSynchronize(b), id);
+        float ms = 0;
+        TG_HIP(hipEventElapsedTime(&ms, a, b), id);
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        *ms_per_launch = ms / reps;
+    });
+}
+
+}  // extern "C"

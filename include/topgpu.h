@@ -214,6 +214,59 @@ int tg_solver_last_timing(const tg_solver *s, double *ms_total);
  * around `reps` back-to-back launches; precision 0 = FP64 operator, 1 = FP32 V-cycle operator. */
 int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t reps, double *ms_per_launch);
 
+/* ---------------------------------------------------------------------------------------
+ * Layer 2: objective, sensitivities, filters and the OC update (device arrays in the
+ * reference's element order; `stream` orders the work).
+ * ------------------------------------------------------------------------------------- */
+
+/* TopOpti_ComputeUnitCompliance (TOP_GPU.m:580-600): ce_e = u_e' Ke u_e with the unmodified Ke;
+ * super-element mode: TopOpti_ComputeUnitCompliance_Super (TOP_GPU.m:2622-2645), u_e' K_mat_e u_e.
+ * d_U: [3Nn] interleaved; d_ce: [Ne].  Synchronises. */
+int tg_solver_unit_compliance(tg_solver *s, const double *d_U, double *d_ce);
+/* pre_compute_der (TOP_GPU.m:2646-2671): d_dc(q, e) = u_e' I_q u_e, [64 x Ne] (super-element mode). */
+int tg_solver_subvoxel_energy(tg_solver *s, const double *d_U, double *d_dc);
+
+/* SIMP modulus E = Emin + x.^p (E0 - Emin)  (TOP_GPU.m:168, 172). */
+int tg_simp_modulus(const double *d_x, int64_t n, double Emin, double E0, double penal, double *d_E, void *stream);
+/* dc = -(p (E0 - Emin) x.^(p-1)) .* (ce / cSolid), ce indexed by i / per  (TOP_GPU.m:169-198, 2641-2643). */
+int tg_sensitivity(const double *d_x, const double *d_ce, int64_t n, int32_t per, double Emin, double E0,
+                   double penal, double cSolid, double *d_dc, void *stream);
+
+enum { TG_EW_MUL = 0, TG_EW_DIV_FLOOR = 1 };
+/* out = a .* b (TG_EW_MUL) or a ./ max(scalar, b) (TG_EW_DIV_FLOOR)  (ft = 1 filtering, TOP_GPU.m:214). */
+int tg_elementwise(int32_t op, const double *d_a, const double *d_b, double scalar, int64_t n, double *d_out,
+                   void *stream);
+
+enum { TG_RED_SUM = 0, TG_RED_DOT = 1, TG_RED_MAXABSDIFF = 2, TG_RED_SHARPNESS = 3 };
+/* Deterministic reductions: sum(a), a.b, max|a-b|, sum(a.*(1-a)); result to host *out.  Synchronises. */
+int tg_reduce(int32_t op, const double *d_a, const double *d_b, int64_t n, double *out, void *stream);
+
+/* Optimality-criteria update with bisection on the Lagrange multiplier (TOP_GPU.m:233-253):
+ * l1 = 0, l2 = 1e9, move; xnew = max(0,max(x-move,min(1,min(x+move,x.*sqrt(-dc./dv/lmid))))),
+ * gt = fval + mean(xnew - x).  Writes d_xnew, *change = max|xnew - x|, *bisections = steps run
+ * (a multiple of 16; trailing steps are no-ops).  Synchronises. */
+int tg_oc_update(const double *d_x, const double *d_dc, const double *d_dv, int64_t n, double fval, double move,
+                 double *d_xnew, double *change, int32_t *bisections, void *stream);
+
+/* PDE (Helmholtz) density filter on the finest grid (TOP_GPU.m:880-1031). */
+typedef struct tg_filter tg_filter;
+int tg_filter_create(const tg_model *m, double rMin, void *stream, tg_filter **out);
+void tg_filter_destroy(tg_filter *f);
+/* tar = A K_F^{-1} A' src: element->node spread, Jacobi-PCG to `tol`, node->element average.
+ * The reference calls it with tol = tol_/10 and maxIT = maxIT_ (TOP_GPU.m:1001).  Synchronises. */
+int tg_filter_apply(tg_filter *f, const double *d_src, double *d_tar, double tol, int32_t max_it, int32_t *its);
+
+/* Super-element density filter: imfilter with the cone max(0, r - |d|) on the num-times refined box,
+ * zero boundary (TOP_GPU.m:112-121, 220-227, 262-265); arrays are [num^3 x Ne] element-major. */
+typedef struct tg_convfilter tg_convfilter;
+int tg_convfilter_create(int32_t nelx, int32_t nely, int32_t nelz, double rMin_subvoxels, int32_t num, void *stream,
+                         tg_convfilter **out);
+void tg_convfilter_destroy(tg_convfilter *f);
+/* xPhys = imfilter(x, h) ./ Hs */
+int tg_convfilter_forward(tg_convfilter *f, const double *d_x, double *d_out);
+/* dc = imfilter(dc ./ Hs, h) */
+int tg_convfilter_adjoint(tg_convfilter *f, const double *d_dc, double *d_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,50 @@
+"""Write the oracle's optimisation histories for the small reference cases as golden fixtures.
+
+    python tests/golden/make_c1_history.py
+
+c1_history.json: TOP_GPU(true(24,48,24), 'optCase',1, 'V0',0.12) -- 50 iterations, ft = 1,
+rMin = sqrt(3), Emin = 1e-6 (the code's value, TOP_GPU.m:390), FP64 -- the compliance,
+volume, sharpness, constraint and change histories, the PCG iteration counts and the final
+xPhys checksum, computed by the oracle (oracle/topopt.py run_top).
+"""
+import hashlib
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import mesh as M  # noqa: E402
+from oracle import topopt as T  # noqa: E402
+
+
+def history(shape, opt_case, V0, rMin, nLoop, ft, super_element=False):
+    m = M.build_cuboid_model(*shape, opt_case=opt_case, super_element=super_element)
+    t = time.time()
+    out = T.run_top(m, V0=V0, rMin=rMin, nLoop=nLoop, ft=ft)
+    xp = np.asarray(out['xPhys']).reshape(-1, order='F')
+    return {
+        'case': dict(shape=list(shape), optCase=opt_case, V0=V0, rMin=rMin, nLoop=nLoop, ft=ft,
+                     super_element=super_element, Emin=1e-6, precision='fp64'),
+        'cSolid': out['cSolid'], 'cHist': out['cHist'], 'volHist': out['volHist'], 'sharpHist': out['sharpHist'],
+        'consHist': out['consHist'], 'changeHist': out['changeHist'], 'lssIts': out['lssIts'],
+        'xPhys_sum': float(xp.sum()), 'xPhys_sha256_f32': hashlib.sha256(xp.astype(np.float32).tobytes()).hexdigest(),
+        'oracle_seconds': time.time() - t,
+    }
+
+
+def main():
+    here = os.path.dirname(os.path.abspath(__file__))
+    h = history((24, 48, 24), 1, 0.12, math.sqrt(3), 50, 1)
+    with open(os.path.join(here, 'c1_history.json'), 'w') as f:
+        json.dump(h, f, indent=1)
+    print('c1: %d iterations, c = %.10e, %.0fs' % (len(h['cHist']), h['cHist'][-1], h['oracle_seconds']))
+
+
+if __name__ == '__main__':
+    main()

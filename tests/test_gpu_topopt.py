@@ -1,0 +1,136 @@
+"""GPU parity of the optimisation layer (compliance, sensitivities, filters, OC update, the full
+driver) against the oracle and the committed C1 golden history."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import mesh as M
+from oracle import solver as S
+from oracle import topopt as T
+from tests.conftest import ROOT, smooth_modulus
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _t(a):
+    import torch
+    return torch.as_tensor(np.asarray(a, dtype=np.float64).reshape(-1, order='F')).cuda()
+
+
+@pytest.mark.parametrize('se', [False, True])
+def test_unit_compliance_and_subvoxel_energy(se):
+    from top_gpu_on_matlab_amd import top_gpu as G
+    from top_gpu_on_matlab_amd.model import cuboid_model
+    from top_gpu_on_matlab_amd.solver import Solver
+    shape, case = ((10, 20, 10), 2) if se else ((8, 16, 8), 1)
+    om = M.build_cuboid_model(*shape, opt_case=case, super_element=se)
+    gm = cuboid_model(*shape, case, super_element=se)
+    E = smooth_modulus(om.levels[0], per=64 if se else 1)
+    sv = Solver(gm)
+    sv.set_modulus(_t(E))
+    sv.assemble()
+    U = np.random.default_rng(0).standard_normal(3 * om.levels[0].numNodes)
+    ce = G.unit_compliance(sv, _t(U)).cpu().numpy()
+    if se:
+        osv = S.Solver(om)
+        osv.assemble(E)
+        assert _rel(ce, T.unit_compliance_super(om.levels[0], U, osv.K_mat)) < 1e-13
+        dc = G.subvoxel_energy(sv, _t(U)).cpu().numpy().reshape(64, -1, order='F')
+        assert _rel(dc, T.pre_compute_der(om.levels[0], U, om.I_mat)) < 1e-13
+    else:
+        assert _rel(ce, T.unit_compliance(om.levels[0], U, om.levels[0].Ke)) < 1e-13
+
+
+def test_pde_filter():
+    from top_gpu_on_matlab_amd import top_gpu as G
+    from top_gpu_on_matlab_amd.model import cuboid_model
+    shape = (12, 24, 12)
+    om = M.build_cuboid_model(*shape, opt_case=1)
+    gm = cuboid_model(*shape, 1)
+    src = np.random.default_rng(1).random(om.levels[0].numElements)
+    for r, tol in ((math.sqrt(3), 1e-4), (3.0, 1e-12)):
+        of = T.PDEFilter(om.levels[0], r)
+        of.tol = tol
+        gf = G.PDEFilter(gm, r, tol=tol)
+        ref = of(src)
+        got = gf(_t(src)).cpu().numpy()
+        assert gf.last_its == of.last_its
+        assert _rel(got, ref) < (1e-9 if tol > 1e-6 else 1e-11)
+
+
+def test_convolution_filter():
+    from top_gpu_on_matlab_amd import top_gpu as G
+    nelx, nely, nelz = 12, 6, 5
+    cf = T.ConvolutionFilter(nelx, nely, nelz, math.sqrt(3))
+    gf = G.ConvolutionFilter(nelx, nely, nelz, 4 * math.sqrt(3))
+    x = np.random.default_rng(2).random((64, nelx * nely * nelz))
+    assert _rel(gf.forward(_t(x)).cpu().numpy(), cf.forward(x).reshape(-1, order='F')) < 1e-13
+    assert _rel(gf.adjoint(_t(x)).cpu().numpy(), cf.adjoint(x).reshape(-1, order='F')) < 1e-13
+
+
+def test_oc_update_matches_bisection():
+    from top_gpu_on_matlab_amd import top_gpu as G
+    rng = np.random.default_rng(3)
+    n = 5000
+    x = 0.05 + 0.9 * rng.random(n)
+    dc = -rng.random(n) * 1e-2
+    dv = 0.5 + rng.random(n)
+    fval = 0.01
+    l1, l2, move = 0.0, 1e9, 0.2
+    while (l2 - l1) / (l1 + l2) > 1e-6:
+        lmid = 0.5 * (l2 + l1)
+        xnew = np.maximum(0, np.maximum(x - move, np.minimum(1, np.minimum(x + move, x * np.sqrt(-dc / dv / lmid)))))
+        if fval + np.mean(xnew - x) > 0:
+            l1 = lmid
+        else:
+            l2 = lmid
+    got, change = G.oc_update(_t(x), _t(dc), _t(dv), fval, move)
+    assert _rel(got.cpu().numpy(), xnew) < 1e-14
+    assert abs(change - np.max(np.abs(xnew - x))) < 1e-15
+
+
+@pytest.mark.parametrize('cfg', [
+    dict(shape=(8, 16, 8), case=1, ft=1, nLoop=6, se=False),
+    dict(shape=(8, 16, 8), case=1, ft=2, nLoop=6, se=False),
+    dict(shape=(10, 20, 10), case=2, ft=2, nLoop=3, se=True),
+], ids=['cantilever-ft1', 'cantilever-ft2', 'mbb-super'])
+def test_optimisation_loop_matches_oracle(cfg):
+    from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
+    print('\n[loop %s]' % cfg, flush=True)
+    om = M.build_cuboid_model(*cfg['shape'], opt_case=cfg['case'], super_element=cfg['se'])
+    ref = T.run_top(om, V0=0.12, rMin=math.sqrt(3), nLoop=cfg['nLoop'], ft=cfg['ft'])
+    got = TOP_GPU(np.ones(cfg['shape'], bool), optCase=cfg['case'], V0=0.12, nLoop=cfg['nLoop'], ft=cfg['ft'],
+                  super_element=int(cfg['se']))
+    assert got['lssIts'] == ref['lssIts']
+    assert abs(got['cSolid'] - ref['cSolid']) < 1e-12 * ref['cSolid']
+    for k in ('cHist', 'volHist', 'sharpHist'):
+        assert _rel(got[k], ref[k]) < 1e-10, (k, got[k], ref[k])
+    assert _rel(got['xPhys'].cpu().numpy(), np.asarray(ref['xPhys']).reshape(-1, order='F')) < 1e-9
+
+
+def test_c1_fifty_iteration_history_against_golden():
+    """TOP_GPU(true(24,48,24),'optCase',1,'V0',0.12): 50 iterations, FP64, against the oracle's history."""
+    from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
+    path = os.path.join(ROOT, 'tests', 'golden', 'c1_history.json')
+    g = json.load(open(path))
+    c = g['case']
+    got = TOP_GPU(np.ones(c['shape'], bool), optCase=c['optCase'], V0=c['V0'], rMin=c['rMin'], nLoop=c['nLoop'],
+                  ft=c['ft'], verbose=True)
+    n = len(g['cHist'])
+    assert len(got['cHist']) == n
+    assert got['lssIts'] == g['lssIts']
+    assert abs(got['cSolid'] - g['cSolid']) < 1e-12 * g['cSolid']
+    err = np.abs(np.array(got['cHist']) - np.array(g['cHist'])) / np.array(g['cHist'])
+    print('max rel compliance-history error vs oracle: %.3e' % err.max(), flush=True)
+    assert err.max() < 1e-10
+    assert _rel(got['volHist'], g['volHist']) < 1e-10
+    assert _rel(got['sharpHist'], g['sharpHist']) < 1e-10
+    assert abs(float(got['xPhys'].sum()) - g['xPhys_sum']) < 1e-9 * g['xPhys_sum']

@@ -13,6 +13,8 @@
 
 #include <cstdint>
 
+#include "kunit.h"
+
 namespace tg {
 
 constexpr uint8_t NODE_ACTIVE = 8;
@@ -80,11 +82,13 @@ __device__ inline void epilogue(T *__restrict__ out, int64_t n, const T acc[3], 
 
 // Finest level, standard voxels: y = sum_e E_e * K0 * u_e with fixed DOFs as identity rows and
 // zero columns (the BC-modified element matrices of the reference, TOP_GPU.m:704-720, 1729-1741).
-// K0 is column-major 24x24; its indices are compile-time constants after unrolling (scalar loads).
-template <typename T, int EPI>
+// CK: K0 = cs * KUNIT_NU03 with the unit table folded into compile-time constants (no loads);
+// otherwise K0 is read from memory (column-major 24x24, cs = 1).  Void and out-of-grid elements
+// carry E = 0 and contribute exactly zero, so the loop over the 8 elements is branch-free.
+template <typename T, int EPI, bool CK>
 __global__ __launch_bounds__(256) void k_kfine(const T *__restrict__ u, T *__restrict__ out,
                                                const T *__restrict__ E, const uint8_t *__restrict__ mask,
-                                               const T *__restrict__ K0, const Grid g, const T *__restrict__ r,
+                                               const T *__restrict__ K0, T cs, const Grid g, const T *__restrict__ r,
                                                const T *__restrict__ diag, T omega) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodes()) return;
@@ -111,25 +115,35 @@ __global__ __launch_bounds__(256) void k_kfine(const T *__restrict__ u, T *__res
                 const bool ok = (unsigned)ey < (unsigned)g.EY && (unsigned)ex < (unsigned)g.EX &&
                                 (unsigned)ez < (unsigned)g.EZ;
                 const T Ee = ok ? E[ey + (int64_t)g.EY * (ex + (int64_t)g.EX * ez)] : (T)0;
-                if (Ee != (T)0) {
-                    const int a = vtx_of(-oy, -ox, -oz);
-                    T s0 = 0, s1 = 0, s2 = 0;
+                const int a = vtx_of(-oy, -ox, -oz);
+                T s0 = 0, s1 = 0, s2 = 0;
 #pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        const int k = nb_of(oy + vtx_y(b), ox + vtx_x(b), oz + vtx_z(b));
+                for (int b = 0; b < 8; ++b) {
+                    const int k = nb_of(oy + vtx_y(b), ox + vtx_x(b), oz + vtx_z(b));
 #pragma unroll
-                        for (int d = 0; d < 3; ++d) {
-                            const T uv = un[k][d];
-                            s0 += uv * K0[(3 * b + d) + 24 * (3 * a + 0)];
-                            s1 += uv * K0[(3 * b + d) + 24 * (3 * a + 1)];
-                            s2 += uv * K0[(3 * b + d) + 24 * (3 * a + 2)];
+                    for (int d = 0; d < 3; ++d) {
+                        const T uv = un[k][d];
+                        const int i0 = (3 * b + d) + 24 * (3 * a + 0), i1 = i0 + 24, i2 = i0 + 48;
+                        if (CK) {
+                            s0 += uv * (T)KUNIT_NU03.v[i0];
+                            s1 += uv * (T)KUNIT_NU03.v[i1];
+                            s2 += uv * (T)KUNIT_NU03.v[i2];
+                        } else {
+                            s0 += uv * K0[i0];
+                            s1 += uv * K0[i1];
+                            s2 += uv * K0[i2];
                         }
                     }
-                    acc[0] += s0 * Ee;
-                    acc[1] += s1 * Ee;
-                    acc[2] += s2 * Ee;
                 }
+                acc[0] += s0 * Ee;
+                acc[1] += s1 * Ee;
+                acc[2] += s2 * Ee;
             }
+    if (CK) {
+        acc[0] *= cs;
+        acc[1] *= cs;
+        acc[2] *= cs;
+    }
     if (mk & 1) acc[0] = u[3 * n];
     if (mk & 2) acc[1] = u[3 * n + 1];
     if (mk & 4) acc[2] = u[3 * n + 2];

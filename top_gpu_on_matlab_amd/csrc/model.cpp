@@ -3,6 +3,8 @@
 // the maps are bit-identical to the reference's unique/intersect/find construction.
 #include "model.h"
 
+#include "kunit.h"
+
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -55,20 +57,12 @@ static void bdb_at(double xi, double eta, double zeta, double nu, double *K) {
         }
 }
 
-// Ke = cellSize * sum over the 2x2x2 Gauss points of B^T D B detJ: equal to the closed form of
-// FEA_VoxelBasedElementStiffnessMatrix (TOP_GPU.m:1465-1503), since the integrand is exactly
+// Ke = cellSize * (2x2x2 Gauss quadrature of B^T D B detJ, kunit.h): equal to the closed form of
+// FEA_VoxelBasedElementStiffnessMatrix (TOP_GPU.m:1465-1503), the integrand being exactly
 // integrated by the 2-point rule.
 void element_stiffness(double nu, double cell, double *Ke) {
-    std::fill(Ke, Ke + 576, 0.0);
-    double g = 1.0 / std::sqrt(3.0), tmp[576];
-    const double pts[2] = {-g, g};
-    for (double xi : pts)
-        for (double eta : pts)
-            for (double zeta : pts) {
-                bdb_at(xi, eta, zeta, nu, tmp);
-                for (int k = 0; k < 576; ++k) Ke[k] += tmp[k];
-            }
-    for (int k = 0; k < 576; ++k) Ke[k] *= cell;
+    const KUnit K = make_kunit(nu);
+    for (int k = 0; k < 576; ++k) Ke[k] = K.v[k] * cell;
 }
 
 // computeK_ij (TOP_GPU.m:2589-2608): sub-voxel q = y + num*(x + num*z), sampled at the

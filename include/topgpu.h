@@ -213,6 +213,12 @@ int tg_solver_last_timing(const tg_solver *s, double *ms_total);
 /* Average duration (ms) of one matrix-free K*u kernel launch at `level`, timed with HIP events
  * around `reps` back-to-back launches; precision 0 = FP64 operator, 1 = FP32 V-cycle operator. */
 int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t reps, double *ms_per_launch);
+/* Launch tuning of the finest-level standard-voxel K*u (no reference counterpart: the reference's
+ * K*u is a MATLAB pagemtimes, TOP_GPU.m:1729-1741).  Keys:
+ *   TG_TUNE_KFINE_KERNEL  0 = element-centric LDS-tiled z-march (default), 1 = node-centric gather
+ *   TG_TUNE_KFINE_ZCHUNK  node planes per workgroup of the tiled kernel (1..4096, default 16) */
+enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2 };
+int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value);
 
 /* ---------------------------------------------------------------------------------------
  * Layer 2: objective, sensitivities, filters and the OC update (device arrays in the

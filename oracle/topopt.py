@@ -159,7 +159,7 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
         return penal * (E0 - Emin) * xp ** (penal - 1)
 
     out = dict(cHist=[], volHist=[], sharpHist=[], consHist=[], lssIts=[], changeHist=[], cSolid=cSolid,
-               tHist=[])
+               tHist=[], resTail=[])
     loop, change, sharp = 0, 1.0, 1.0
     while loop < nLoop and change > 0.0001 and sharp > 0.01:
         t0 = time.perf_counter()
@@ -167,9 +167,10 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
         E = simp(xPhys)
         sv.assemble(E)
         t1 = time.perf_counter()
-        U, its, _ = sv.pcg(F, tol, maxIT, y0=U)
+        U, its, res = sv.pcg(F, tol, maxIT, y0=U)
         t2 = time.perf_counter()
         out['lssIts'].append(its)
+        out['resTail'].append([float(v) for v in res[-2:]])   # relative residuals deciding the stop
         if not se:
             ce = unit_compliance(lv1, U, lv1.Ke)
             ceNorm = ce / cSolid

@@ -5,7 +5,9 @@
 c1_history.json: TOP_GPU(true(24,48,24), 'optCase',1, 'V0',0.12) -- 50 iterations, ft = 1,
 rMin = sqrt(3), Emin = 1e-6 (the code's value, TOP_GPU.m:390), FP64 -- the compliance,
 volume, sharpness, constraint and change histories, the PCG iteration counts and the final
-xPhys checksum, computed by the oracle (oracle/topopt.py run_top).
+xPhys checksum, computed by the oracle (oracle/topopt.py run_top).  resTail holds the last two
+relative PCG residuals of every solve: an implementation whose rounding differs may stop one
+iteration earlier or later only where one of them is within rounding distance of tol.
 """
 import hashlib
 import json
@@ -33,6 +35,7 @@ def history(shape, opt_case, V0, rMin, nLoop, ft, super_element=False):
                      super_element=super_element, Emin=1e-6, precision='fp64'),
         'cSolid': out['cSolid'], 'cHist': out['cHist'], 'volHist': out['volHist'], 'sharpHist': out['sharpHist'],
         'consHist': out['consHist'], 'changeHist': out['changeHist'], 'lssIts': out['lssIts'],
+        'resTail': out['resTail'], 'tol': 1e-3,
         'xPhys_sum': float(xp.sum()), 'xPhys_sha256_f32': hashlib.sha256(xp.astype(np.float32).tobytes()).hexdigest(),
         'oracle_seconds': time.time() - t,
     }

@@ -86,6 +86,7 @@ SIGNATURES = {
     'tg_solver_get_ks': (ctypes.c_int, [_VP, _I32, _VP]),
     'tg_solver_last_timing': (ctypes.c_int, [_VP, ctypes.POINTER(_D)]),
     'tg_solver_time_kbyu': (ctypes.c_int, [_VP, _I32, _I32, _I32, ctypes.POINTER(_D)]),
+    'tg_solver_set_tuning': (ctypes.c_int, [_VP, _I32, _I32]),
     'tg_solver_unit_compliance': (ctypes.c_int, [_VP, _VP, _VP]),
     'tg_solver_subvoxel_energy': (ctypes.c_int, [_VP, _VP, _VP]),
     'tg_simp_modulus': (ctypes.c_int, [_VP, _I64, _D, _D, _D, _VP, _VP]),
@@ -102,6 +103,7 @@ SIGNATURES = {
     'tg_convfilter_adjoint': (ctypes.c_int, [_VP, _VP, _VP]),
 }
 TG_EW_MUL, TG_EW_DIV_FLOOR = 0, 1
+TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK = 1, 2
 TG_RED_SUM, TG_RED_DOT, TG_RED_MAXABSDIFF, TG_RED_SHARPNESS = 0, 1, 2, 3
 
 _lib = None

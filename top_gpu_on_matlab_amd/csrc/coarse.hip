@@ -75,7 +75,8 @@ __global__ void k_pack_op(const double *__restrict__ Lc, const double *__restric
     W[i] = v;
 }
 
-// out[r] = sum_c W[r, c] * in(c), in = [in1 (n1) ; in2 (n2)], one wavefront per row.
+// out[r] = sum_c W[r, c] * in(c), in = [in1 (n1) ; in2 (n2)], one wavefront per row.  Eight
+// independent loads per lane are in flight per step (eight partial sums, combined in a fixed order).
 __global__ __launch_bounds__(256) void k_gemv_rows(const double *__restrict__ W, int64_t rows, int64_t n1, int64_t n2,
                                                    const double *__restrict__ in1, const double *__restrict__ in2,
                                                    double *__restrict__ out) {
@@ -84,11 +85,27 @@ __global__ __launch_bounds__(256) void k_gemv_rows(const double *__restrict__ W,
     if (r >= rows) return;
     const int64_t w = n1 + n2;
     const double *row = W + r * w;
-    double s = 0;
-    for (int64_t c = lane; c < n1; c += 64) s += row[c] * in1[c];
-    for (int64_t c = lane; c < n2; c += 64) s += row[n1 + c] * in2[c];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-    if (lane == 0) out[r] = s;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int64_t c = lane;
+    for (; c + 7 * 64 < w; c += 8 * 64) {
+        double a[8], x[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int64_t cc = c + 64 * k;
+            a[k] = row[cc];
+            x[k] = cc < n1 ? in1[cc] : in2[cc - n1];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] += a[k] * x[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+        const int64_t cc = c + 64 * k;
+        if (cc < w) s[k] += row[cc] * (cc < n1 ? in1[cc] : in2[cc - n1]);
+    }
+    double t = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
+    if (lane == 0) out[r] = t;
 }
 
 template <typename T>

@@ -80,6 +80,14 @@ __device__ inline void epilogue(T *__restrict__ out, int64_t n, const T acc[3], 
     }
 }
 
+// Workgroup barrier that orders LDS accesses only: outstanding global loads stay in flight across
+// it (__syncthreads() also waits for them), which lets a z-march prefetch the next plane.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Finest level, standard voxels: y = sum_e E_e * K0 * u_e with fixed DOFs as identity rows and
 // zero columns (the BC-modified element matrices of the reference, TOP_GPU.m:704-720, 1729-1741).
 // CK: K0 = cs * KUNIT_NU03 with the unit table folded into compile-time constants (no loads);
@@ -150,15 +158,51 @@ __global__ __launch_bounds__(256) void k_kfine(const T *__restrict__ u, T *__res
     epilogue<T, EPI>(out, n, acc, r, diag, u, omega);
 }
 
-// Per-element matrices in entry-major ("SoA") layout: Km[i * ldk + e], i = r + 24c.
-// Coarse levels (Galerkin Ks, MASKED = false) and the super-element finest level
-// (K_mat = I_mat * E_sub with BC masks, MASKED = true).  y_e = u_e^T * K_e (TOP_GPU.m:757-760).
+// Per-element matrices in entry-major ("SoA") layout: Km[i * ldk + e], i = r + 24c, for the coarse
+// levels (Galerkin Ks, MASKED = false) and the super-element finest level (K_mat = I_mat * E_sub
+// with BC masks, MASKED = true); y_e = u_e^T * K_e (TOP_GPU.m:757-760).
+// Two stages.  Stage 1: y_e = u_e^T K_e for every solid
+// element, four threads per element (6 output columns each; consecutive threads take consecutive
+// elements, so every matrix entry row is streamed coalesced), written entry-major
+// (f[j * ldk + e]).  Stage 2: each node sums its (up to 8) element contributions in the fixed
+// element order (z, x, y), then applies identity rows for fixed DOFs (MASKED) and the epilogue.
+constexpr int KE_PARTS = 4;
+template <typename T, bool MASKED>
+__global__ __launch_bounds__(256) void k_kelem_vec(const T *__restrict__ u, T *__restrict__ f,
+                                                   const T *__restrict__ Km, int64_t ldk,
+                                                   const uint8_t *__restrict__ eleMask,
+                                                   const uint8_t *__restrict__ mask, const Grid g) {
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int part = (int)(gid / ldk);
+    const int64_t e = gid - (int64_t)part * ldk;
+    if (part >= KE_PARTS || e >= g.elems() || !eleMask[e]) return;
+    const int ey = (int)(e % g.EY);
+    const int64_t t = e / g.EY;
+    const int ex = (int)(t % g.EX), ez = (int)(t / g.EX);
+    T ue[24];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int64_t n = (ey + vtx_y(b)) + (int64_t)g.NY * ((ex + vtx_x(b)) + (int64_t)g.NX * (ez + vtx_z(b)));
+        const uint8_t mm = MASKED ? mask[n] : (uint8_t)0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ue[3 * b + d] = (mm & (1 << d)) ? (T)0 : u[3 * n + d];
+    }
+    const T *K = Km + e;
+#pragma unroll
+    for (int jj = 0; jj < 24 / KE_PARTS; ++jj) {
+        const int j = part * (24 / KE_PARTS) + jj;
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < 24; ++i) s += ue[i] * K[(int64_t)(i + 24 * j) * ldk];
+        f[(int64_t)j * ldk + e] = s;
+    }
+}
+
 template <typename T, int EPI, bool MASKED>
-__global__ __launch_bounds__(256) void k_kelem(const T *__restrict__ u, T *__restrict__ out,
-                                               const T *__restrict__ Km, int64_t ldk,
-                                               const uint8_t *__restrict__ eleMask,
-                                               const uint8_t *__restrict__ mask, const Grid g,
-                                               const T *__restrict__ r, const T *__restrict__ diag, T omega) {
+__global__ __launch_bounds__(256) void k_kelem_gather(const T *__restrict__ f, int64_t ldk, T *__restrict__ out,
+                                                      const T *__restrict__ u, const uint8_t *__restrict__ eleMask,
+                                                      const uint8_t *__restrict__ mask, const Grid g,
+                                                      const T *__restrict__ r, const T *__restrict__ diag, T omega) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodes()) return;
     const int iy = (int)(n % g.NY);
@@ -171,8 +215,6 @@ __global__ __launch_bounds__(256) void k_kelem(const T *__restrict__ u, T *__res
         out[3 * n + 2] = 0;
         return;
     }
-    T un[27][3];
-    load_neighbours<T, MASKED>(un, u, mask, g, iy, ix, iz, n);
     T acc[3] = {0, 0, 0};
 #pragma unroll
     for (int oz = -1; oz <= 0; ++oz)
@@ -186,22 +228,9 @@ __global__ __launch_bounds__(256) void k_kelem(const T *__restrict__ u, T *__res
                 const int64_t e = ok ? ey + (int64_t)g.EY * (ex + (int64_t)g.EX * ez) : 0;
                 if (ok && eleMask[e]) {
                     const int a = vtx_of(-oy, -ox, -oz);
-                    T s0 = 0, s1 = 0, s2 = 0;
-#pragma unroll
-                    for (int b = 0; b < 8; ++b) {
-                        const int k = nb_of(oy + vtx_y(b), ox + vtx_x(b), oz + vtx_z(b));
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) {
-                            const T uv = un[k][d];
-                            const int row = 3 * b + d;
-                            s0 += uv * Km[(int64_t)(row + 24 * (3 * a + 0)) * ldk + e];
-                            s1 += uv * Km[(int64_t)(row + 24 * (3 * a + 1)) * ldk + e];
-                            s2 += uv * Km[(int64_t)(row + 24 * (3 * a + 2)) * ldk + e];
-                        }
-                    }
-                    acc[0] += s0;
-                    acc[1] += s1;
-                    acc[2] += s2;
+                    acc[0] += f[(int64_t)(3 * a) * ldk + e];
+                    acc[1] += f[(int64_t)(3 * a + 1) * ldk + e];
+                    acc[2] += f[(int64_t)(3 * a + 2) * ldk + e];
                 }
             }
     if (MASKED) {
@@ -225,85 +254,98 @@ __device__ inline T ri_w(int a, int b, int c, int j) {
 // Restriction r_c = R r_f (Solving_RestrictResidual, TOP_GPU.m:817-847): for every coarse node J,
 // sum over the non-empty coarse elements around J of RI(:,J)' * (r_f ./ coeff) on the element's
 // (S+1)^3 fine nodes.  coeff holds transferMatCoeffi per fine grid node (1 where unused).
+// Eight lanes per coarse node, one per adjacent coarse element (lane k: element offset
+// (k&1, k>>1&1, k>>2) - 1); each walks the S^3 fine nodes where J's trilinear weight is nonzero,
+// and the eight partial sums are combined in a fixed tree order.
 template <typename T, int S>
 __global__ __launch_bounds__(256) void k_restrict(const T *__restrict__ rf, T *__restrict__ rc,
                                                   const T *__restrict__ coeff, const uint8_t *__restrict__ cMask,
                                                   const uint8_t *__restrict__ cEle, const Grid gc, const Grid gf) {
-    const int64_t J = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (J >= gc.nodes()) return;
-    const int jy = (int)(J % gc.NY);
-    const int64_t t = J / gc.NY;
-    const int jx = (int)(t % gc.NX), jz = (int)(t / gc.NX);
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t J = gid >> 3;
+    const int sub = (int)(gid & 7);
+    const bool valid = J < gc.nodes();
     T a0 = 0, a1 = 0, a2 = 0;
-    if (cMask[J] & NODE_ACTIVE) {
-        for (int oz = -1; oz <= 0; ++oz)
-            for (int ox = -1; ox <= 0; ++ox)
-                for (int oy = -1; oy <= 0; ++oy) {
-                    const int ey = jy + oy, ex = jx + ox, ez = jz + oz;
-                    if ((unsigned)ey >= (unsigned)gc.EY || (unsigned)ex >= (unsigned)gc.EX ||
-                        (unsigned)ez >= (unsigned)gc.EZ)
-                        continue;
-                    if (!cEle[ey + (int64_t)gc.EY * (ex + (int64_t)gc.EX * ez)]) continue;
-                    const int jv = vtx_of(-oy, -ox, -oz);
-                    for (int c = 0; c <= S; ++c)
-                        for (int b = 0; b <= S; ++b)
+    if (valid && (cMask[J] & NODE_ACTIVE)) {
+        const int jy = (int)(J % gc.NY);
+        const int64_t t = J / gc.NY;
+        const int jx = (int)(t % gc.NX), jz = (int)(t / gc.NX);
+        const int oy = (sub & 1) - 1, ox = ((sub >> 1) & 1) - 1, oz = (sub >> 2) - 1;
+        const int ey = jy + oy, ex = jx + ox, ez = jz + oz;
+        if ((unsigned)ey < (unsigned)gc.EY && (unsigned)ex < (unsigned)gc.EX && (unsigned)ez < (unsigned)gc.EZ &&
+            cEle[ey + (int64_t)gc.EY * (ex + (int64_t)gc.EX * ez)]) {
+            const int jv = vtx_of(-oy, -ox, -oz);
+            const int vy = vtx_y(jv), vx = vtx_x(jv), vz = vtx_z(jv);  // nonzero weights: offsets v..v+S-1
+            const int64_t base = (int64_t)(S * ey + vy) + (int64_t)gf.NY * ((S * ex + vx) + (int64_t)gf.NX * (S * ez + vz));
 #pragma unroll
-                            for (int a = 0; a <= S; ++a) {
-                                const T w = ri_w<T, S>(a, b, c, jv);
-                                if (w == (T)0) continue;
-                                const int64_t i = (int64_t)(S * ey + a) +
-                                                  (int64_t)gf.NY * ((S * ex + b) + (int64_t)gf.NX * (S * ez + c));
-                                const T cf = coeff[i];
-                                a0 += w * (rf[3 * i] / cf);
-                                a1 += w * (rf[3 * i + 1] / cf);
-                                a2 += w * (rf[3 * i + 2] / cf);
-                            }
-                }
+            for (int c = 0; c < S; ++c)
+#pragma unroll
+                for (int b = 0; b < S; ++b)
+#pragma unroll
+                    for (int a = 0; a < S; ++a) {
+                        const T w = ri_w<T, S>(a + vy, b + vx, c + vz, jv);
+                        const int64_t i = base + a + (int64_t)gf.NY * (b + (int64_t)gf.NX * c);
+                        const T cf = coeff[i];
+                        a0 += w * (rf[3 * i] / cf);
+                        a1 += w * (rf[3 * i + 1] / cf);
+                        a2 += w * (rf[3 * i + 2] / cf);
+                    }
+        }
     }
-    rc[3 * J] = a0;
-    rc[3 * J + 1] = a1;
-    rc[3 * J + 2] = a2;
+#pragma unroll
+    for (int o = 1; o < 8; o <<= 1) {
+        a0 += __shfl_down(a0, o, 8);
+        a1 += __shfl_down(a1, o, 8);
+        a2 += __shfl_down(a2, o, 8);
+    }
+    if (valid && sub == 0) {
+        rc[3 * J] = a0;
+        rc[3 * J + 1] = a1;
+        rc[3 * J + 2] = a2;
+    }
 }
 
-// Prolongation x_f (+)= P x_c (Solving_InterpolationDeviation, TOP_GPU.m:848-878): every mesh
-// node of the fine grid averages the trilinear interpolants of the non-empty coarse elements
-// that contain it (sum, then divide by transferMatCoeffi).
+// Prolongation x_f (+)= P x_c (Solving_InterpolationDeviation, TOP_GPU.m:848-878): the reference
+// sums the trilinear interpolants of every non-empty coarse element containing a fine node and
+// divides by their count (transferMatCoeffi).  The trilinear interpolant is continuous across
+// element faces, so all those interpolants are the same number: each active fine node takes the
+// interpolant of the first non-empty coarse element containing it (z, x, y order).
 template <typename T, int S, bool ADD>
 __global__ __launch_bounds__(256) void k_prolong(const T *__restrict__ xc, T *__restrict__ xf,
-                                                 const T *__restrict__ coeff, const uint8_t *__restrict__ fMask,
-                                                 const uint8_t *__restrict__ cEle, const Grid gc, const Grid gf) {
+                                                 const uint8_t *__restrict__ fMask, const uint8_t *__restrict__ cEle,
+                                                 const Grid gc, const Grid gf) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= gf.nodes()) return;
     const int iy = (int)(i % gf.NY);
     const int64_t t = i / gf.NY;
     const int ix = (int)(t % gf.NX), iz = (int)(t / gf.NX);
     T a0 = 0, a1 = 0, a2 = 0;
-    const bool act = fMask[i] & NODE_ACTIVE;
-    if (act) {
-        const int qy = iy / S, qx = ix / S, qz = iz / S;
-        const int ny0 = (iy % S == 0) ? qy - 1 : qy, nx0 = (ix % S == 0) ? qx - 1 : qx, nz0 = (iz % S == 0) ? qz - 1 : qz;
-        for (int ez = nz0; ez <= qz; ++ez)
-            for (int ex = nx0; ex <= qx; ++ex)
-                for (int ey = ny0; ey <= qy; ++ey) {
-                    if ((unsigned)ey >= (unsigned)gc.EY || (unsigned)ex >= (unsigned)gc.EX ||
-                        (unsigned)ez >= (unsigned)gc.EZ)
-                        continue;
-                    if (!cEle[ey + (int64_t)gc.EY * (ex + (int64_t)gc.EX * ez)]) continue;
-                    const int a = iy - S * ey, b = ix - S * ex, c = iz - S * ez;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const int64_t Jn = (int64_t)(ey + vtx_y(j)) +
-                                           (int64_t)gc.NY * ((ex + vtx_x(j)) + (int64_t)gc.NX * (ez + vtx_z(j)));
-                        const T w = ri_w<T, S>(a, b, c, j);
-                        a0 += w * xc[3 * Jn];
-                        a1 += w * xc[3 * Jn + 1];
-                        a2 += w * xc[3 * Jn + 2];
+    if (fMask[i] & NODE_ACTIVE) {
+        // coarse elements containing the node: e in [ceil(i/S) - 1, floor(i/S)], clamped to the grid
+        const int qy = min(iy / S, gc.EY - 1), qx = min(ix / S, gc.EX - 1), qz = min(iz / S, gc.EZ - 1);
+        const int ny0 = max((iy + S - 1) / S - 1, 0), nx0 = max((ix + S - 1) / S - 1, 0);
+        const int nz0 = max((iz + S - 1) / S - 1, 0);
+        int ey = -1, ex = -1, ez = -1;
+        for (int cz = nz0; cz <= qz && ez < 0; ++cz)
+            for (int cx = nx0; cx <= qx && ez < 0; ++cx)
+                for (int cy = ny0; cy <= qy && ez < 0; ++cy)
+                    if (cEle[cy + (int64_t)gc.EY * (cx + (int64_t)gc.EX * cz)]) {
+                        ey = cy;
+                        ex = cx;
+                        ez = cz;
                     }
-                }
-        const T cf = coeff[i];
-        a0 = a0 / cf;
-        a1 = a1 / cf;
-        a2 = a2 / cf;
+        if (ez >= 0) {
+            const int a = iy - S * ey, b = ix - S * ex, c = iz - S * ez;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int64_t Jn = (int64_t)(ey + vtx_y(j)) +
+                                   (int64_t)gc.NY * ((ex + vtx_x(j)) + (int64_t)gc.NX * (ez + vtx_z(j)));
+                const T w = ri_w<T, S>(a, b, c, j);
+                a0 += w * xc[3 * Jn];
+                a1 += w * xc[3 * Jn + 1];
+                a2 += w * xc[3 * Jn + 2];
+            }
+        }
     }
     if (ADD) {
         xf[3 * i] += a0;

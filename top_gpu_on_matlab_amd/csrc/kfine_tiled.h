@@ -1,0 +1,315 @@
+// Finest-level K*u for standard voxels: element-centric, LDS-tiled z-march, with the element
+// matrix applied in the Walsh (+-1 Hadamard) basis of the 8 vertices.
+//
+// Walsh factorisation.  Number the vertices q = vy + 2 vx + 4 vz and let W[q][m] = prod over the
+// axes a in m of (2 q_a - 1) (the 8 modes: constant, the three linear, three bilinear and the
+// trilinear function, evaluated at the vertices).  W^T W = 8 I, so for the 24x24 element matrix
+//     K0 = T KW T^T,   T = W (x) I_3,   KW = T^T K0 T / 64.
+// For any isotropic Hex8 matrix KW has 45 nonzeros of 576 (the constant mode carries no energy and
+// the cube's symmetries decouple most mode pairs), so f_e = E_e K0 u_e costs two 8-point butterflies
+// per component (72 adds each way), 45 FMAs and 21 scalings instead of 576 FMAs.
+//
+// Tiling.  A workgroup owns a KT_TY x KT_TX column of nodes (y, x) over a chunk of z node planes
+// and marches through the element layers between them.  Per element layer:
+//   1. the next node plane of the (KT_TY+2) x (KT_TX+2) halo box (fetched into registers during
+//      the previous layer) is committed to LDS, fixed DOFs zeroed (the zero columns of the
+//      BC-modified element matrices, TOP_GPU.m:704-720), and the plane after it is fetched;
+//   2. each of the 256 lanes computes the element vector of one element of the (KT_TY+1) x
+//      (KT_TX+1) layer and stores it in LDS;
+//   3. each owned node adds the 4 lower-vertex contributions of this layer to the upper-vertex
+//      contributions it kept from the previous layer (fixed order: deterministic), applies identity
+//      rows for fixed DOFs and the epilogue (K*u, r - K*u or the damped-Jacobi update), and
+//      writes its 3 values once; it then keeps this layer's 4 upper-vertex contributions.
+#pragma once
+
+#include "kernels.h"
+
+namespace tg {
+
+constexpr int KT_TY = 15, KT_TX = 15;                // owned nodes per plane (225 of 256 lanes)
+constexpr int KT_PY = KT_TY + 2, KT_PX = KT_TX + 2;  // staged nodes per plane, with the halo
+constexpr int KT_EY = KT_TY + 1, KT_EX = KT_TX + 1;  // elements per layer: 256, one per lane
+constexpr int KT_PN = KT_PY * KT_PX;
+constexpr int KT_NI = (3 * KT_PN + 255) / 256;        // staged values per lane and plane
+static_assert(KT_EY * KT_EX == 256, "one element per lane");
+
+// ---- Walsh basis ---------------------------------------------------------------------------
+__host__ __device__ constexpr int vtx_q(int b) { return vtx_y(b) + 2 * vtx_x(b) + 4 * vtx_z(b); }
+__host__ __device__ constexpr int walsh_sign(int q, int m) {
+    int s = 1;
+    for (int a = 0; a < 3; ++a)
+        if ((m >> a) & 1) s *= ((q >> a) & 1) ? 1 : -1;
+    return s;
+}
+
+// KW = T^T K0 T / 64 (column-major over index 3m + c), entries below 1e-13 max|KW| set to zero.
+// Two passes of 8-term sums (A = K0 T, then T^T A) keep the compile-time evaluation short.
+inline constexpr KUnit make_kwalsh(const KUnit &K) {
+    int sg[8][8] = {};  // sg[b][m] = W[q(b)][m]
+    for (int b = 0; b < 8; ++b)
+        for (int m = 0; m < 8; ++m) sg[b][m] = walsh_sign(vtx_q(b), m);
+    KUnit A{}, w{};
+    for (int j = 0; j < 24; ++j)
+        for (int i = 0; i < 24; ++i) {
+            double s = 0;
+            for (int b = 0; b < 8; ++b) s += K.v[i + 24 * (3 * b + j % 3)] * sg[b][j / 3];
+            A.v[i + 24 * j] = s;
+        }
+    double mx = 0;
+    for (int j = 0; j < 24; ++j)
+        for (int i = 0; i < 24; ++i) {
+            double s = 0;
+            for (int b = 0; b < 8; ++b) s += sg[b][i / 3] * A.v[(3 * b + i % 3) + 24 * j];
+            s /= 64;
+            w.v[i + 24 * j] = s;
+            mx = s > mx ? s : (-s > mx ? -s : mx);
+        }
+    for (int k = 0; k < 576; ++k)
+        if (w.v[k] < 1e-13 * mx && w.v[k] > -1e-13 * mx) w.v[k] = 0;
+    return w;
+}
+
+struct KWPattern {
+    int idx[576];  // compact position of entry (i, j) = i + 24 j, or -1
+    int n;
+};
+inline constexpr KWPattern make_kwpattern(const KUnit &kw) {
+    KWPattern p{};
+    p.n = 0;
+    for (int k = 0; k < 576; ++k) p.idx[k] = kw.v[k] != 0 ? p.n++ : -1;
+    return p;
+}
+constexpr KUnit KWALSH_NU03 = make_kwalsh(KUNIT_NU03);
+__host__ __device__ constexpr KWPattern KW_PATTERN = make_kwpattern(KWALSH_NU03);
+constexpr int KW_NNZ = 45;
+static_assert(KW_PATTERN.n == KW_NNZ, "isotropic Hex8 matrix: 45 nonzeros in the Walsh basis");
+
+template <typename T>
+struct KWVals {
+    T v[48];  // the KW_NNZ pattern entries of cs * KW, in pattern order
+};
+
+// ---- tiles -----------------------------------------------------------------------------------
+struct KTiles {
+    int tilesY, tilesX, chunks, zchunk;
+    __host__ __device__ int64_t count() const { return (int64_t)tilesY * tilesX * chunks; }
+};
+
+inline KTiles ktiles_for(const Grid &g, int zchunk) {
+    KTiles t;
+    t.tilesY = (g.NY + KT_TY - 1) / KT_TY;
+    t.tilesX = (g.NX + KT_TX - 1) / KT_TX;
+    t.zchunk = zchunk;
+    t.chunks = (g.NZ + zchunk - 1) / zchunk;
+    return t;
+}
+
+// 8-point Walsh butterflies over q: forward m = W^T x, inverse x = W g.
+template <typename T>
+__device__ __forceinline__ void walsh_fwd(T x[8]) {
+#pragma unroll
+    for (int s = 1; s < 8; s <<= 1)
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (!(i & s)) {
+                const T a = x[i], b = x[i | s];
+                x[i] = a + b;
+                x[i | s] = b - a;
+            }
+}
+template <typename T>
+__device__ __forceinline__ void walsh_inv(T x[8]) {
+#pragma unroll
+    for (int s = 1; s < 8; s <<= 1)
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (!(i & s)) {
+                const T a = x[i], b = x[i | s];
+                x[i] = a - b;
+                x[i | s] = a + b;
+            }
+}
+
+// CK: KW is the compile-time unit matrix for nu = 0.3 and each element is scaled by cs * E_e;
+// otherwise KW (cellSize included) comes from kw and cs is ignored.
+template <typename T, int EPI, bool CK>
+__global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) void k_kfine_walsh(const T *__restrict__ u, T *__restrict__ out,
+                                                     const T *__restrict__ E, const uint8_t *__restrict__ mask,
+                                                     const KWVals<T> kw, T cs, const Grid g, const KTiles kt,
+                                                     const T *__restrict__ r, const T *__restrict__ diag, T omega) {
+    __shared__ T sU[2][3][KT_PN];
+    __shared__ uint8_t sM[2][KT_PN];
+    // FP64 element vectors pass through LDS in two halves (lower, then upper vertices) so that four
+    // workgroups fit in a CU's LDS; FP32 ones in one pass.
+    constexpr bool SPLIT = sizeof(T) == 8;
+    __shared__ T sF[SPLIT ? 12 : 24][256];
+    const int t = threadIdx.x;
+    int bid = blockIdx.x;
+    const int tpc = kt.tilesY * kt.tilesX;
+    const int tz = bid / tpc;
+    bid -= tz * tpc;
+    const int ty = bid % kt.tilesY, tx = bid / kt.tilesY;
+    const int y0 = ty * KT_TY, x0 = tx * KT_TX;
+    const int z0 = tz * kt.zchunk;
+    const int z1 = min(z0 + kt.zchunk, g.NZ);
+    const int64_t NYX = (int64_t)g.NY * g.NX;
+    const int64_t EYX = (int64_t)g.EY * g.EX;
+
+    // element lane
+    const int ly = t % KT_EY, lx = t / KT_EY;
+    const int ey = y0 - 1 + ly, ex = x0 - 1 + lx;
+    const bool eIn = (unsigned)ey < (unsigned)g.EY && (unsigned)ex < (unsigned)g.EX;
+    const T *Ecol = E + (eIn ? ey + (int64_t)g.EY * ex : 0);
+    // node lane (lanes >= KT_TY * KT_TX own none)
+    const int oy = t % KT_TY, ox = t / KT_TY;
+    const bool owner = t < KT_TY * KT_TX && y0 + oy < g.NY && x0 + ox < g.NX;
+
+    // Prefetch: loads are unconditional (clamped addresses) and all masking is applied at commit
+    // time, so no wait for the loaded values is needed until the next layer.
+    T pre[KT_NI];
+    uint8_t preM[KT_NI];
+    bool preOk[KT_NI];
+    auto fetch = [&](int z) {
+#pragma unroll
+        for (int k = 0; k < KT_NI; ++k) {
+            const int i = min(t + 256 * k, 3 * KT_PN - 1);
+            const int node = i / 3, c = i - 3 * node;
+            const int py = node % KT_PY, px = node / KT_PY;
+            const int gy = y0 - 1 + py, gx = x0 - 1 + px;
+            const bool ok = (unsigned)gy < (unsigned)g.NY && (unsigned)gx < (unsigned)g.NX && (unsigned)z < (unsigned)g.NZ;
+            const int64_t n = ok ? gy + (int64_t)g.NY * gx + NYX * z : 0;
+            pre[k] = u[3 * n + c];
+            preM[k] = mask[n];
+            preOk[k] = ok;
+        }
+    };
+    auto commit = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < KT_NI; ++k) {
+            const int i = t + 256 * k;
+            if (i < 3 * KT_PN) {
+                const int node = i / 3, c = i - 3 * node;
+                const uint8_t mk = preOk[k] ? preM[k] : (uint8_t)0;
+                sU[slot][c][node] = (preOk[k] && !(mk & (1 << c))) ? pre[k] : (T)0;
+                if (c == 0) sM[slot][node] = mk;
+            }
+        }
+    };
+    T Eraw = 0;
+    bool Eok = false;
+    auto fetchE = [&](int layer) {
+        Eok = eIn && (unsigned)layer < (unsigned)g.EZ;
+        Eraw = Ecol[Eok ? EYX * layer : 0];
+    };
+
+    fetch(z0 - 1);
+    commit(0);
+    fetch(z0);
+    fetchE(z0 - 1);
+    T part[3] = {0, 0, 0};  // upper-vertex contributions to the owned node of the next plane
+    int cur = 0;
+    for (int ez = z0 - 1; ez < z1; ++ez) {
+        commit(cur ^ 1);  // node plane ez + 1
+        const T Ee = Eok ? Eraw : (T)0;
+        lds_barrier();
+        // the owned node's mask (slot cur is overwritten once every lane has passed the next barrier)
+        const uint8_t mk = owner ? sM[cur][(oy + 1) + KT_PY * (ox + 1)] : (uint8_t)0;
+        if (ez + 1 < z1) {
+            fetch(ez + 2);
+            fetchE(ez + 1);
+        }
+        // ---- element vector of this lane's element
+        T fv[24];
+        {
+            const T Es = CK ? cs * Ee : Ee;
+            T m[24];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T x[8];
+#pragma unroll
+                for (int b = 0; b < 8; ++b) {
+                    const int pn = (ly + vtx_y(b)) + KT_PY * (lx + vtx_x(b));
+                    x[vtx_q(b)] = sU[vtx_z(b) ? cur ^ 1 : cur][c][pn];
+                }
+                walsh_fwd(x);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) m[3 * q + c] = x[q];
+            }
+            T gq[24];
+#pragma unroll
+            for (int i = 0; i < 24; ++i) {
+                T acc = 0;
+#pragma unroll
+                for (int j = 0; j < 24; ++j)
+                    if (KW_PATTERN.idx[i + 24 * j] >= 0)
+                        acc += (CK ? (T)KWALSH_NU03.v[i + 24 * j] : kw.v[KW_PATTERN.idx[i + 24 * j]]) * m[j];
+                gq[i] = acc * Es;
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T x[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) x[q] = gq[3 * q + c];
+                walsh_inv(x);
+#pragma unroll
+                for (int b = 0; b < 8; ++b) fv[3 * b + c] = x[vtx_q(b)];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < (SPLIT ? 12 : 24); ++j) sF[j][t] = fv[j];
+        lds_barrier();
+        // ---- owned node: complete plane ez, start plane ez + 1
+        T up[3] = {0, 0, 0};
+        if (owner) {
+            T lo[3] = {part[0], part[1], part[2]};
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy) {
+                    const int te = (oy + dy) + KT_EY * (ox + dx);
+                    const int bl = vtx_of(1 - dy, 1 - dx, 0), bu = bl + 4;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        lo[d] += sF[3 * bl + d][te];
+                        if (!SPLIT) up[d] += sF[3 * bu + d][te];
+                    }
+                }
+            if (ez >= z0) {
+                const int64_t n = (y0 + oy) + (int64_t)g.NY * (x0 + ox) + NYX * ez;
+                if (mk & NODE_ACTIVE) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        if (mk & (1 << d)) lo[d] = u[3 * n + d];
+                    epilogue<T, EPI>(out, n, lo, r, diag, u, omega);
+                } else {
+                    out[3 * n] = 0;
+                    out[3 * n + 1] = 0;
+                    out[3 * n + 2] = 0;
+                }
+            }
+        }
+        if (SPLIT) {
+            lds_barrier();
+#pragma unroll
+            for (int j = 0; j < 12; ++j) sF[j][t] = fv[12 + j];
+            lds_barrier();
+            if (owner) {
+#pragma unroll
+                for (int dx = 0; dx < 2; ++dx)
+#pragma unroll
+                    for (int dy = 0; dy < 2; ++dy) {
+                        const int te = (oy + dy) + KT_EY * (ox + dx);
+                        const int bl = vtx_of(1 - dy, 1 - dx, 0);
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) up[d] += sF[3 * bl + d][te];
+                    }
+            }
+        }
+        part[0] = up[0];
+        part[1] = up[1];
+        part[2] = up[2];
+        cur ^= 1;
+    }
+}
+
+}  // namespace tg

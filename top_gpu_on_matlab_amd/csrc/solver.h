@@ -3,6 +3,7 @@
 
 #include "coarse.h"
 #include "kernels.h"
+#include "kfine_tiled.h"
 #include "model.h"
 #include "pcg.h"
 
@@ -18,6 +19,8 @@ struct DevLevel {
     DevBuf<float> diag32, r32, x32, t32;
     DevBuf<double> Ks64;                   // element matrices, entry-major [576 x ldk]
     DevBuf<float> Ks32;
+    DevBuf<double> fv64;                   // per-element K*u products, entry-major [24 x ldk]
+    DevBuf<float> fv32;
     DevBuf<double> coeffF64;               // transferMatCoeffi on the finer grid's nodes (levels >= 2)
     DevBuf<float> coeffF32;
     DevBuf<double> P;                      // Pch / Psub [24 x 24 x span^3] (levels >= 2)
@@ -32,6 +35,11 @@ struct Solver {
     double omega = 0.6;
     bool adapted = false, super = false;
     bool const_k0 = false;  // nu == 0.3: finest K*u uses the compile-time unit element matrix
+    int kfine_kernel = 0;   // 0: LDS-tiled element-centric (Walsh basis), 1: node-centric gather
+    bool walsh_ok = false;  // the finest element matrix has the isotropic Walsh-basis pattern
+    KWVals<double> kw64{};
+    KWVals<float> kw32{};
+    int kfine_zchunk = 16;  // node planes per workgroup of the tiled kernel
     bool has_modulus = false, assembled = false;
     std::vector<DevLevel> lv;
     CoarseSolver coarse;

@@ -9,6 +9,7 @@
 #include "coarse.h"
 #include "galerkin.h"
 #include "kernels.h"
+#include "kfine_tiled.h"
 #include "model.h"
 #include "pcg.h"
 #include "solver.h"
@@ -221,8 +222,12 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
         }
         if (i >= 1) {
             D.Ks64.alloc(576 * D.ldk);
+            D.fv64.alloc(24 * D.ldk);
             D.Ks64.zero(st);
-            if (prec == TG_PREC_MIXED) D.Ks32.alloc(576 * D.ldk);
+            if (prec == TG_PREC_MIXED) {
+                D.Ks32.alloc(576 * D.ldk);
+                D.fv32.alloc(24 * D.ldk);
+            }
             // transfer coefficients on the finer grid (1 where a node is unused)
             const HostLevel &F = m.lv[i - 1];
             std::vector<double> cf(F.nodesFull(), 1.0);
@@ -240,6 +245,21 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
     }
     DevLevel &F = lv[0];
     K0_64.upload(m.Ke, st);
+    {
+        // Walsh-basis form of the finest element matrix for the tiled K*u (isotropic: 45 entries)
+        KUnit k0{};
+        for (int i = 0; i < 576; ++i) k0.v[i] = m.Ke[i];
+        const KUnit kw = make_kwalsh(k0);
+        walsh_ok = true;
+        for (int i = 0; i < 576; ++i) {
+            const int c = KW_PATTERN.idx[i];
+            if (c < 0 && kw.v[i] != 0) walsh_ok = false;
+            if (c >= 0) {
+                kw64.v[c] = kw.v[i];
+                kw32.v[c] = (float)kw.v[i];
+            }
+        }
+    }
     if (prec == TG_PREC_MIXED) {
         std::vector<float> k32(m.Ke.begin(), m.Ke.end());
         K0_32.upload(k32, st);
@@ -255,8 +275,12 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
         Esub.alloc(64 * nEf);
         Esub.zero(st);
         F.Ks64.alloc(576 * F.ldk);  // K_mat on the finest level
+        F.fv64.alloc(24 * F.ldk);
         F.Ks64.zero(st);
-        if (prec == TG_PREC_MIXED) F.Ks32.alloc(576 * F.ldk);
+        if (prec == TG_PREC_MIXED) {
+            F.Ks32.alloc(576 * F.ldk);
+            F.fv32.alloc(24 * F.ldk);
+        }
     } else {
         E64.alloc(nEf);
         E64.zero(st);
@@ -353,6 +377,21 @@ void Solver::assemble() {
 }
 
 template <typename T>
+static void launch_walsh(int epi, bool ck, unsigned gt, hipStream_t st, const T *u, T *out, const T *E,
+                         const uint8_t *mask, const KWVals<T> &kw, T cs, const Grid &g, const KTiles &kt,
+                         const T *r, const T *d, T w) {
+    if (ck) {
+        if (epi == EPI_KU) k_kfine_walsh<T, EPI_KU, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
+        else if (epi == EPI_RESID) k_kfine_walsh<T, EPI_RESID, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
+        else k_kfine_walsh<T, EPI_JACOBI, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
+    } else {
+        if (epi == EPI_KU) k_kfine_walsh<T, EPI_KU, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
+        else if (epi == EPI_RESID) k_kfine_walsh<T, EPI_RESID, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
+        else k_kfine_walsh<T, EPI_JACOBI, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
+    }
+}
+
+template <typename T>
 void Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T *dg) {
     DevLevel &D = lv[i];
     const unsigned gr = grid_for(D.g.nodes(), 256);
@@ -362,7 +401,24 @@ void Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
         const T *E = f64 ? (const T *)E64.p : (const T *)E32.p;
         const T *K0 = f64 ? (const T *)K0_64.p : (const T *)K0_32.p;
         const T cs = (T)model.prm.cell_size;
-        if (const_k0) {
+        if (kfine_kernel == 0 && walsh_ok) {
+            const KTiles kt = ktiles_for(D.g, kfine_zchunk);
+            const unsigned gt = (unsigned)kt.count();
+            if (f64) {
+                const KWVals<double> &k = kw64;
+                const double *uu = (const double *)u, *r2 = (const double *)rr, *d2 = (const double *)dg;
+                const double *E2 = (const double *)E;
+                double *o2 = (double *)out;
+                launch_walsh<double>(epi, const_k0, gt, st, uu, o2, E2, D.nodeMask, k, model.prm.cell_size, D.g, kt, r2, d2, omega);
+            } else {
+                const KWVals<float> &k = kw32;
+                const float *uu = (const float *)u, *r2 = (const float *)rr, *d2 = (const float *)dg;
+                const float *E2 = (const float *)E;
+                float *o2 = (float *)out;
+                const float w32 = (float)omega;
+                launch_walsh<float>(epi, const_k0, gt, st, uu, o2, E2, D.nodeMask, k, (float)model.prm.cell_size, D.g, kt, r2, d2, w32);
+            }
+        } else if (const_k0) {
             if (epi == EPI_KU) k_kfine<T, EPI_KU, true><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, cs, D.g, rr, dg, w);
             else if (epi == EPI_RESID) k_kfine<T, EPI_RESID, true><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, cs, D.g, rr, dg, w);
             else k_kfine<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, cs, D.g, rr, dg, w);
@@ -373,14 +429,18 @@ void Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
         }
     } else {
         const T *Km = f64 ? (const T *)D.Ks64.p : (const T *)D.Ks32.p;
+        T *fv = f64 ? (T *)D.fv64.p : (T *)D.fv32.p;
+        const unsigned ge = grid_for(KE_PARTS * D.ldk, 256);
         if (i == 0) {
-            if (epi == EPI_KU) k_kelem<T, EPI_KU, true><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
-            else if (epi == EPI_RESID) k_kelem<T, EPI_RESID, true><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
-            else k_kelem<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            k_kelem_vec<T, true><<<ge, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else k_kelem_gather<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
         } else {
-            if (epi == EPI_KU) k_kelem<T, EPI_KU, false><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
-            else if (epi == EPI_RESID) k_kelem<T, EPI_RESID, false><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
-            else k_kelem<T, EPI_JACOBI, false><<<gr, 256, 0, st>>>(u, out, Km, D.ldk, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            k_kelem_vec<T, false><<<ge, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else k_kelem_gather<T, EPI_JACOBI, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
         }
     }
     TG_LAUNCH_CHECK("tg_solver:kbyu");
@@ -390,7 +450,7 @@ template <typename T>
 void Solver::restrict_level(int c, const T *rf, T *rc) {
     DevLevel &C = lv[c], &F = lv[c - 1];
     const T *cf = sizeof(T) == 8 ? (const T *)C.coeffF64.p : (const T *)C.coeffF32.p;
-    const unsigned gr = grid_for(C.g.nodes(), 256);
+    const unsigned gr = grid_for(8 * C.g.nodes(), 256);
     if (C.span == 4) k_restrict<T, 4><<<gr, 256, 0, st>>>(rf, rc, cf, C.nodeMask, C.eleMask, C.g, F.g);
     else k_restrict<T, 2><<<gr, 256, 0, st>>>(rf, rc, cf, C.nodeMask, C.eleMask, C.g, F.g);
     TG_LAUNCH_CHECK("tg_solver:restrict");
@@ -399,14 +459,13 @@ void Solver::restrict_level(int c, const T *rf, T *rc) {
 template <typename T>
 void Solver::prolong_level(int c, const T *xc, T *xf, bool add) {
     DevLevel &C = lv[c], &F = lv[c - 1];
-    const T *cf = sizeof(T) == 8 ? (const T *)C.coeffF64.p : (const T *)C.coeffF32.p;
     const unsigned gr = grid_for(F.g.nodes(), 256);
     if (C.span == 4) {
-        if (add) k_prolong<T, 4, true><<<gr, 256, 0, st>>>(xc, xf, cf, F.nodeMask, C.eleMask, C.g, F.g);
-        else k_prolong<T, 4, false><<<gr, 256, 0, st>>>(xc, xf, cf, F.nodeMask, C.eleMask, C.g, F.g);
+        if (add) k_prolong<T, 4, true><<<gr, 256, 0, st>>>(xc, xf, F.nodeMask, C.eleMask, C.g, F.g);
+        else k_prolong<T, 4, false><<<gr, 256, 0, st>>>(xc, xf, F.nodeMask, C.eleMask, C.g, F.g);
     } else {
-        if (add) k_prolong<T, 2, true><<<gr, 256, 0, st>>>(xc, xf, cf, F.nodeMask, C.eleMask, C.g, F.g);
-        else k_prolong<T, 2, false><<<gr, 256, 0, st>>>(xc, xf, cf, F.nodeMask, C.eleMask, C.g, F.g);
+        if (add) k_prolong<T, 2, true><<<gr, 256, 0, st>>>(xc, xf, F.nodeMask, C.eleMask, C.g, F.g);
+        else k_prolong<T, 2, false><<<gr, 256, 0, st>>>(xc, xf, F.nodeMask, C.eleMask, C.g, F.g);
     }
     TG_LAUNCH_CHECK("tg_solver:prolong");
 }
@@ -706,6 +765,16 @@ extern "C" {
 
 // Average duration of one K*u kernel launch at `level` (precision 0: FP64, 1: FP32 V-cycle copy),
 // timed with HIP events around `reps` back-to-back launches on the solver's stream.
+int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
+    return guard("tg_solver_set_tuning:Input", [&] {
+        const char *id = "tg_solver_set_tuning:Input";
+        Solver &S = S_of(s, id);
+        if (key == TG_TUNE_KFINE_KERNEL && (value == 0 || value == 1)) S.kfine_kernel = value;
+        else if (key == TG_TUNE_KFINE_ZCHUNK && value >= 1 && value <= 4096) S.kfine_zchunk = value;
+        else fail(TG_ERR_ARGS, id, "unknown tuning key or value out of range");
+    });
+}
+
 int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t reps, double *ms_per_launch) {
     return guard("tg_solver_time_kbyu:Input", [&] {
         const char *id = "tg_solver_time_kbyu:Input";

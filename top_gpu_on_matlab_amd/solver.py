@@ -118,6 +118,14 @@ class Solver:
                                         ctypes.byref(ms)))
         return ms.value
 
+    def set_tuning(self, kfine_kernel=None, kfine_zchunk=None):
+        """Finest-level K*u launch choice: kernel 'tiled' (default) or 'gather'; z planes per tile."""
+        from ._lib import TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK
+        if kfine_kernel is not None:
+            check(lib().tg_solver_set_tuning(self.h, TG_TUNE_KFINE_KERNEL, {'tiled': 0, 'gather': 1}[kfine_kernel]))
+        if kfine_zchunk is not None:
+            check(lib().tg_solver_set_tuning(self.h, TG_TUNE_KFINE_ZCHUNK, int(kfine_zchunk)))
+
     def last_pcg_ms(self):
         ms = ctypes.c_double()
         check(lib().tg_solver_last_timing(self.h, ctypes.byref(ms)))

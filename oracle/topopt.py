@@ -159,7 +159,7 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
         return penal * (E0 - Emin) * xp ** (penal - 1)
 
     out = dict(cHist=[], volHist=[], sharpHist=[], consHist=[], lssIts=[], changeHist=[], cSolid=cSolid,
-               tHist=[], resTail=[])
+               tHist=[], resTail=[], ocMargin=[])
     loop, change, sharp = 0, 1.0, 1.0
     while loop < nLoop and change > 0.0001 and sharp > 0.01:
         t0 = time.perf_counter()
@@ -201,10 +201,12 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
         t4 = time.perf_counter()
         fval = float(np.mean(xPhys)) - V0
         l1, l2, move = 0.0, 1e9, 0.2
+        margin = math.inf   # smallest |fval + mean(xnew - x)| the bisection decided on
         while (l2 - l1) / (l1 + l2) > 1e-6:
             lmid = 0.5 * (l2 + l1)
             xnew = np.maximum(0, np.maximum(x - move, np.minimum(1, np.minimum(x + move, x * np.sqrt(-dc / dv / lmid)))))
             gt = fval + float(np.mean(xnew - x))
+            margin = min(margin, abs(gt))
             if gt > 0:
                 l1 = lmid
             else:
@@ -224,6 +226,7 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
         out['cHist'].append(cDesign)
         out['volHist'].append(V)
         out['consHist'].append(fval)
+        out['ocMargin'].append(margin)
         out['sharpHist'].append(sharp)
         out['changeHist'].append(change)
         out['tHist'].append([t1 - t0, t2 - t1, (t3 - t2) + (t5 - t4), (t4 - t3) + (t6 - t5), t6 - t0])

@@ -128,10 +128,12 @@ def test_vcycle(pair):
 
 
 def test_pcg_parity_and_determinism(pair):
-    """PCG on a smooth density field: same iteration count as the oracle, compliance b'y to 1e-9 and
-    the iterate to 1e-8 (max-abs/max).  The two differ only by rounding (summation order in the
-    element products, transfers and coarse solve); the perforated grid amplifies it most (measured:
-    iterate 3e-9, compliance 5e-10; the solid box: compliance below 1e-12)."""
+    """PCG on a smooth density field against the oracle.  The device and the oracle differ only by
+    rounding (summation order in element products, transfers and the coarse solve), which CG
+    amplifies; the bound is calibrated on the oracle itself: a relative perturbation of b at the
+    unit-roundoff level (2^-52) moves the oracle's compliance b'y and iterate by some spread, and
+    the device must agree with the oracle to within 20x that spread (never looser than 1e-8 for the
+    compliance) with the same iteration count."""
     name, om, osv, gm, gsv, E = pair
     Es = smooth_modulus(om.levels[0], per=64 if om.super_element else 1)
     osv = S.Solver(om)
@@ -140,16 +142,24 @@ def test_pcg_parity_and_determinism(pair):
     gsv.assemble()
     b = om.F[:, 0]
     y_ref, its_ref, hist = osv.pcg(b, tol=1e-3, maxIT=800)
+    bp = b * (1 + 2.0 ** -52 * np.random.default_rng(11).standard_normal(b.shape))
+    y_per, its_per, _ = osv.pcg(bp, tol=1e-3, maxIT=800)
+    c_ref = float(b @ y_ref)
+    c_spread = abs(float(b @ y_per) - c_ref) / abs(c_ref)
+    y_spread = _rel(y_per, y_ref)
     y, its, res = gsv.pcg(b, tol=1e-3, max_it=800)
     y2, its2, res2 = gsv.pcg(b, tol=1e-3, max_it=800)
+    c_dev = abs(float(b @ y.cpu().numpy()) - c_ref) / abs(c_ref)
+    print('\n[%s] its %d; oracle ulp-spread: compliance %.1e iterate %.1e; device: compliance %.1e iterate %.1e'
+          % (name, its_ref, c_spread, y_spread, c_dev, _rel(y.cpu().numpy(), y_ref)), flush=True)
     assert its == its_ref, (name, its, its_ref, hist[-3:], res)
-    assert abs(float(b @ y.cpu().numpy()) - float(b @ y_ref)) < 1e-9 * abs(float(b @ y_ref))
-    assert _rel(y.cpu().numpy(), y_ref) < 1e-8
+    assert c_dev < max(20 * c_spread, 1e-12) and c_dev < 1e-8
+    assert _rel(y.cpu().numpy(), y_ref) < max(20 * y_spread, 1e-10)
     assert its2 == its and bool((y == y2).all()), 'PCG is not bitwise reproducible'
     # warm start from the previous solution (TOP_GPU.m:186)
     y3, its3, _ = gsv.pcg(b, y0=y, tol=1e-3, max_it=800)
     y3r, its3r, _ = osv.pcg(b, tol=1e-3, maxIT=800, y0=y.cpu().numpy())
     assert its3 == its3r
-    assert _rel(y3.cpu().numpy(), y3r) < 1e-8
+    assert _rel(y3.cpu().numpy(), y3r) < max(20 * y_spread, 1e-10)
     gsv.set_modulus(np.asarray(E).reshape(-1, order='F'))      # restore the fixture's state
     gsv.assemble()

@@ -119,12 +119,12 @@ def test_optimisation_loop_matches_oracle(cfg):
 def test_c1_fifty_iteration_history_against_golden():
     """TOP_GPU(true(24,48,24),'optCase',1,'V0',0.12): 50 iterations, FP64, against the oracle's history.
 
-    PCG stops on ||r||/||b|| < tol.  Rounding that differs from the oracle's (summation order) can move
-    the stopping step by one where the oracle's deciding residual lies within a few percent of tol
-    (the residual at tol 1e-3 amplifies the ~1e-12 history differences); from such an iteration on
-    the histories legitimately differ at the level of one PCG step.  So: every iteration up to the
-    first count difference must match to 1e-10, and a count difference is accepted only as a near
-    tie (|res/tol - 1| < 2% for the oracle, counts within one)."""
+    The history is chaotic with respect to rounding: the oracle itself, with its load vector
+    perturbed at the unit-roundoff level (the golden's ulpPerturbed run), leaves its own history by
+    ~1e-7 within ten iterations (CG amplifies last-bit differences, and PCG stopping and OC bisection
+    decisions then branch).  So the device must (1) match the oracle to 1e-10, with identical PCG
+    counts, over the iterations where the perturbed oracle still does, and (2) stay within ten times
+    the perturbed oracle's largest deviation over all 50 iterations."""
     from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
     path = os.path.join(ROOT, 'tests', 'golden', 'c1_history.json')
     g = json.load(open(path))
@@ -134,19 +134,17 @@ def test_c1_fifty_iteration_history_against_golden():
     n = len(g['cHist'])
     assert len(got['cHist']) == n
     assert abs(got['cSolid'] - g['cSolid']) < 1e-10 * g['cSolid']
-    upto = n
-    for k in range(n):
-        if got['lssIts'][k] != g['lssIts'][k]:
-            near = min(abs(r / g['tol'] - 1) for r in g['resTail'][k])
-            print('PCG count differs at iteration %d: %d vs %d (oracle residual/tol %s)'
-                  % (k + 1, got['lssIts'][k], g['lssIts'][k], g['resTail'][k]), flush=True)
-            assert abs(got['lssIts'][k] - g['lssIts'][k]) == 1 and near < 0.02
-            upto = k
-            break
-    assert upto >= 40, 'histories must agree over at least 40 of the 50 iterations'
+    up = g['ulpPerturbed']
     for key in ('cHist', 'volHist', 'sharpHist'):
-        err = np.abs(np.array(got[key][:upto]) - np.array(g[key][:upto])) / np.abs(np.array(g[key][:upto]))
-        print('%s: max rel error vs oracle over %d iterations: %.3e' % (key, upto, err.max()), flush=True)
-        assert err.max() < 1e-10, key
-    if upto == n:
-        assert abs(float(got['xPhys'].sum()) - g['xPhys_sum']) < 1e-9 * g['xPhys_sum']
+        ref = np.array(g[key])
+        spread = np.abs(np.array(up[key]) - ref) / np.abs(ref)
+        dev = np.abs(np.array(got[key]) - ref) / np.abs(ref)
+        K = int(np.argmax(spread > 1e-10)) if (spread > 1e-10).any() else n
+        print('%s: oracle ulp-spread max %.2e (first above 1e-10 at iteration %d); device max %.2e, '
+              'over the first %d iterations %.2e' % (key, spread.max(), K + 1, dev.max(), K, dev[:K].max()),
+              flush=True)
+        assert K >= 5
+        assert dev[:K].max() < 1e-10, key
+        assert dev.max() < max(10 * spread.max(), 1e-10), key
+        if key == 'cHist':
+            assert got['lssIts'][:K] == g['lssIts'][:K]

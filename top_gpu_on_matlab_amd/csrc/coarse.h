@@ -4,10 +4,11 @@
 // The coarsest grid is cut into node planes along its longest axis.  With nodes ordered plane
 // by plane, K_c is block tridiagonal; its Cholesky factor is block bidiagonal:
 //   L_pp L_pp^T = A_pp - L_{p,p-1} L_{p,p-1}^T,   L_{p+1,p} = A_{p+1,p} L_pp^{-T}.
-// The factorisation runs on the device (rocSOLVER potrf/trtri, rocBLAS syrk/trsm/gemm).  Each
-// solve is 2P dense GEMV steps with precomputed row-major operators
-//   forward  y_p = [inv(L_pp) | -inv(L_pp) L_{p,p-1}]      * [b_p; y_{p-1}]
-//   backward x_p = [inv(L_pp)^T | -inv(L_pp)^T L_{p+1,p}^T] * [y_p; x_{p+1}].
+// The factorisation runs on the device (rocSOLVER potrf/trtri, rocBLAS syrk/trsm/gemm).  A solve
+// is two batched triangular products (all planes at once) and two chains of P-1 dense GEMV steps:
+//   forward  y = blockdiag(inv(L_pp)) b,    y_p -= inv(L_pp) L_{p,p-1} y_{p-1}
+//   backward x = blockdiag(inv(L_pp)^T) y,  x_p -= inv(L_pp)^T L_{p+1,p}^T x_{p+1}
+// captured once as a HIP graph (its buffers are fixed at setup) and replayed on every solve.
 #pragma once
 
 #include <rocsolver/rocsolver.h>
@@ -22,16 +23,19 @@ struct CoarseSolver {
     int P = 0;                      // number of planes
     std::vector<int64_t> nd;        // DOFs per plane
     std::vector<int64_t> off;       // DOF offset per plane
-    std::vector<int64_t> adOff, asOff, wOff, vOff;  // offsets into the dense buffers
+    std::vector<int64_t> adOff, asOff;  // offsets of the diagonal / sub-diagonal plane blocks
+    std::vector<int64_t> mOff, hOff;    // offsets of the padded forward / backward chain operators
     int64_t ntot = 0;
     Grid g{};
     DevBuf<int32_t> lidx;           // per full-grid node: local node index within its plane (-1 none)
     DevBuf<int64_t> d_off, d_adOff, d_asOff;
-    DevBuf<double> Ad, As, Linv, Fm, Hm, W, V;
+    DevBuf<int32_t> rowPlane;       // plane of each row of the stacked vector
+    DevBuf<double> Ad, As, Linv, LT, Mf, Hm;
     DevBuf<double> bb, yy, xx;
     DevBuf<rocblas_int> info;
     rocblas_handle h = nullptr;
     bool factored = false;
+    hipGraphExec_t exec = nullptr;  // the captured solve sequence
 
     ~CoarseSolver();
     void setup(const HostLevel &L, const Grid &grid, hipStream_t st);
@@ -40,6 +44,7 @@ struct CoarseSolver {
     // x = K_c^{-1} r on full-grid interleaved vectors (T = float or double; solve in FP64).
     template <typename T>
     void solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st);
+    void enqueue_core(hipStream_t s) const;
 };
 
 }  // namespace tg

@@ -57,55 +57,93 @@ __global__ void k_lower_only(const double *__restrict__ S, double *__restrict__ 
     L[i] = (r >= c) ? S[i] : 0.0;
 }
 
-// W (row-major, n x (n + m)) = [A | -B] with A (n x n) read transposed or not, B (n x m).
-//   transposed == 0: A(r, c) = Lc[r + n c],       B(r, c') = Bc[r + n c']        (B col-major n x m)
-//   transposed == 1: A(r, c) = Lc[c + n r],       B(r, c') = Bc[c' + m r]        (B = H^T, H col-major m x n)
-__global__ void k_pack_op(const double *__restrict__ Lc, const double *__restrict__ Bc, double *__restrict__ W,
-                          int64_t n, int64_t m, int transposed) {
+// Row-major copy of a column-major n x n block: Tr[r * n + c] = A[r + n * c].
+__global__ void k_transpose_block(const double *__restrict__ A, double *__restrict__ Tr, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t w = n + m;
-    if (i >= n * w) return;
-    const int64_t r = i / w, c = i % w;
-    double v;
-    if (c < n) v = transposed ? Lc[c + n * r] : Lc[r + n * c];
-    else {
-        const int64_t cc = c - n;
-        v = -(transposed ? Bc[cc + m * r] : Bc[r + n * cc]);
-    }
-    W[i] = v;
+    if (i >= n * n) return;
+    const int64_t r = i / n, c = i - r * n;
+    Tr[i] = A[r + n * c];
 }
 
-// out[r] = sum_c W[r, c] * in(c), in = [in1 (n1) ; in2 (n2)], one wavefront per row.  Eight
-// independent loads per lane are in flight per step (eight partial sums, combined in a fixed order).
-__global__ __launch_bounds__(256) void k_gemv_rows(const double *__restrict__ W, int64_t rows, int64_t n1, int64_t n2,
-                                                   const double *__restrict__ in1, const double *__restrict__ in2,
-                                                   double *__restrict__ out) {
+// Batched triangular products over all planes, one wavefront per row (row R of the stacked
+// vector, plane rowPlane[R]):  out_p = Tri_p * in_p, Tri_p row-major n_p x n_p, lower (columns
+// 0..r) or upper (columns r..n_p-1) triangular.
+__global__ __launch_bounds__(256) void k_tri_rows(const double *__restrict__ Tri, const int64_t *__restrict__ adOff,
+                                                  const int64_t *__restrict__ off, const int32_t *__restrict__ rowPlane,
+                                                  int64_t ntot, int upper, const double *__restrict__ in,
+                                                  double *__restrict__ out) {
     const int lane = threadIdx.x & 63;
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= rows) return;
-    const int64_t w = n1 + n2;
-    const double *row = W + r * w;
+    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (R >= ntot) return;
+    const int p = rowPlane[R];
+    const int64_t o = off[p], n = off[p + 1] - o, r = R - o;
+    const double *row = Tri + adOff[p] + r * n;
+    const double *x = in + o;
+    const int64_t c0 = upper ? r : 0, c1 = upper ? n : r + 1;
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int64_t c = lane;
-    for (; c + 7 * 64 < w; c += 8 * 64) {
-        double a[8], x[8];
+    for (int64_t c = c0 + lane; c < c1; c += 8 * 64) {
+        // loads are unconditional (clamped index) so all eight are in flight before the first use
+        double a[8], b[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int64_t cc = c + 64 * k;
+            const int64_t cc = min(c + 64 * k, c1 - 1);
             a[k] = row[cc];
-            x[k] = cc < n1 ? in1[cc] : in2[cc - n1];
+            b[k] = x[cc];
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) s[k] += a[k] * x[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 7; ++k) {
-        const int64_t cc = c + 64 * k;
-        if (cc < w) s[k] += row[cc] * (cc < n1 ? in1[cc] : in2[cc - n1]);
+        for (int k = 0; k < 8; ++k) s[k] += (c + 64 * k < c1) ? a[k] * b[k] : 0.0;
     }
     double t = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
+    for (int o2 = 32; o2 > 0; o2 >>= 1) t += __shfl_down(t, o2, 64);
+    if (lane == 0) out[R] = t;
+}
+
+// One step of a block recurrence, one wavefront per row:  y[r] = c[r] - sum_j M[r * ldm + j] v[j]
+// (M row-major n x m, rows zero-padded to ldm = nchunks * NK * 64; y may alias c).  The block
+// stages v in LDS chunk by chunk; each lane issues its NK matrix loads of a chunk before the
+// first use (no guards: the padding is zero), so a chunk costs one memory round trip.
+template <int NK>
+__global__ __launch_bounds__(256) void k_chain_rows(const double *__restrict__ M, int64_t n, int64_t m, int64_t ldm,
+                                                    int nchunks, const double *c, const double *__restrict__ v,
+                                                    double *y) {
+    __shared__ double sv[NK * 64];
+    const int lane = threadIdx.x & 63;
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const double *row = M + (r < n ? r : 0) * ldm + lane;
+    double t = 0;
+    for (int q = 0; q < nchunks; ++q) {
+        double a[NK];
+#pragma unroll
+        for (int k = 0; k < NK; ++k) a[k] = row[(int64_t)q * NK * 64 + 64 * k];
+        __syncthreads();
+        for (int j = threadIdx.x; j < NK * 64; j += 256) {
+            const int64_t jj = (int64_t)q * NK * 64 + j;
+            sv[j] = jj < m ? v[jj] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NK; ++k) t += a[k] * sv[lane + 64 * k];
+    }
     for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
-    if (lane == 0) out[r] = t;
+    if (lane == 0 && r < n) y[r] = c[r] - t;
+}
+
+// Padded row length (multiple of 64, or of 1024 beyond 1024) and the chunking used for it.
+inline int64_t chain_ld(int64_t m) { return m <= 1024 ? ((m + 63) / 64) * 64 : ((m + 1023) / 1024) * 1024; }
+
+void launch_chain(const double *M, int64_t n, int64_t m, const double *c, const double *v, double *y, hipStream_t s) {
+    const int64_t ld = chain_ld(m);
+    const unsigned gr = grid_for(n, 4);
+    const int nk = (int)(ld <= 1024 ? ld / 64 : 16), nch = (int)(ld <= 1024 ? 1 : ld / 1024);
+    switch (nk) {
+#define TG_CHAIN_CASE(K) \
+    case K: k_chain_rows<K><<<gr, 256, 0, s>>>(M, n, m, ld, nch, c, v, y); break;
+        TG_CHAIN_CASE(1) TG_CHAIN_CASE(2) TG_CHAIN_CASE(3) TG_CHAIN_CASE(4) TG_CHAIN_CASE(5) TG_CHAIN_CASE(6)
+        TG_CHAIN_CASE(7) TG_CHAIN_CASE(8) TG_CHAIN_CASE(9) TG_CHAIN_CASE(10) TG_CHAIN_CASE(11) TG_CHAIN_CASE(12)
+        TG_CHAIN_CASE(13) TG_CHAIN_CASE(14) TG_CHAIN_CASE(15) TG_CHAIN_CASE(16)
+#undef TG_CHAIN_CASE
+        default: fail(TG_ERR_ARGS, "tg_solver:coarse", "bad chain chunking");
+    }
 }
 
 template <typename T>
@@ -148,6 +186,7 @@ void rb(rocblas_status s, const char *what) {
 }  // namespace
 
 CoarseSolver::~CoarseSolver() {
+    if (exec) (void)hipGraphExecDestroy(exec);
     if (h) rocblas_destroy_handle(h);
 }
 
@@ -180,26 +219,31 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
     ntot = off[P];
     adOff.assign(P + 1, 0);
     asOff.assign(P + 1, 0);
-    wOff.assign(P + 1, 0);
-    vOff.assign(P + 1, 0);
+    mOff.assign(P + 1, 0);
+    hOff.assign(P + 1, 0);
     for (int p = 0; p < P; ++p) {
         const int64_t prev = p > 0 ? nd[p - 1] : 0, next = p + 1 < P ? nd[p + 1] : 0;
         adOff[p + 1] = adOff[p] + nd[p] * nd[p];
         asOff[p + 1] = asOff[p] + nd[p] * prev;
-        wOff[p + 1] = wOff[p] + nd[p] * (nd[p] + prev);
-        vOff[p + 1] = vOff[p] + nd[p] * (nd[p] + next);
+        mOff[p + 1] = mOff[p] + nd[p] * (prev ? chain_ld(prev) : 0);
+        hOff[p + 1] = hOff[p] + nd[p] * (next ? chain_ld(next) : 0);
     }
+    std::vector<int32_t> rp(std::max<int64_t>(ntot, 1), 0);
+    for (int p = 0; p < P; ++p)
+        for (int64_t R = off[p]; R < off[p + 1]; ++R) rp[R] = p;
     lidx.upload(lid, st);
     d_off.upload(off, st);
     d_adOff.upload(adOff, st);
     d_asOff.upload(asOff, st);
+    rowPlane.upload(rp, st);
     Ad.alloc(adOff[P]);
     Linv.alloc(adOff[P]);
+    LT.alloc(adOff[P]);
     As.alloc(std::max<int64_t>(asOff[P], 1));
-    Fm.alloc(std::max<int64_t>(asOff[P], 1));
-    Hm.alloc(std::max<int64_t>(asOff[P], 1));
-    W.alloc(wOff[P]);
-    V.alloc(vOff[P]);
+    Mf.alloc(std::max<int64_t>(mOff[P], 1));
+    Hm.alloc(std::max<int64_t>(hOff[P], 1));
+    Mf.zero(st);  // row padding stays zero
+    Hm.zero(st);
     bb.alloc(ntot);
     yy.alloc(ntot);
     xx.alloc(ntot);
@@ -237,24 +281,24 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
                              rocblas_diagonal_non_unit, (rocblas_int)nd[p + 1], np, &one, S, np, As.p + asOff[p + 1],
                              (rocblas_int)nd[p + 1]), "rocblas_dtrsm");
     }
+    // Solve operators (all row-major): LT_p = inv(L_pp) and UT_p = inv(L_pp)^T (the column-major
+    // Linv itself), Mf_p = inv(L_pp) L_{p,p-1} (n_p x n_{p-1}, from (As_p^T Linv_p^T) column-major)
+    // and Hm_p = inv(L_pp)^T L_{p+1,p}^T (n_p x n_{p+1}, from L_{p+1,p} inv(L_pp) column-major), the
+    // last two with rows zero-padded to chain_ld.
     for (int p = 0; p < P; ++p) {
         const rocblas_int np = (rocblas_int)nd[p];
         if (np == 0) continue;
         const int64_t prev = p > 0 ? nd[p - 1] : 0, next = p + 1 < P ? nd[p + 1] : 0;
+        k_transpose_block<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Linv.p + adOff[p], LT.p + adOff[p], np);
+        TG_LAUNCH_CHECK(id);
         if (prev > 0)
-            rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, np, (rocblas_int)prev, np, &one,
-                             Linv.p + adOff[p], np, As.p + asOff[p], np, &zero, Fm.p + asOff[p], np), "rocblas_dgemm F");
+            rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_transpose, (rocblas_int)prev, np, np,
+                             &one, As.p + asOff[p], np, Linv.p + adOff[p], np, &zero, Mf.p + mOff[p],
+                             (rocblas_int)chain_ld(prev)), "rocblas_dgemm Mf");
         if (next > 0)
             rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)next, np, np, &one,
                              As.p + asOff[p + 1], (rocblas_int)next, Linv.p + adOff[p], np, &zero,
-                             Hm.p + asOff[p + 1], (rocblas_int)next), "rocblas_dgemm H");
-        k_pack_op<<<grid_for(np * (np + prev), 256), 256, 0, st>>>(Linv.p + adOff[p], Fm.p + asOff[p],
-                                                                   W.p + wOff[p], np, prev, 0);
-        TG_LAUNCH_CHECK(id);
-        k_pack_op<<<grid_for(np * (np + next), 256), 256, 0, st>>>(Linv.p + adOff[p],
-                                                                   next > 0 ? Hm.p + asOff[p + 1] : Hm.p,
-                                                                   V.p + vOff[p], np, next, 1);
-        TG_LAUNCH_CHECK(id);
+                             Hm.p + hOff[p], (rocblas_int)chain_ld(next)), "rocblas_dgemm Hm");
     }
     std::vector<rocblas_int> hinfo(2 * P);
     TG_HIP(hipMemcpyAsync(hinfo.data(), info.p, sizeof(rocblas_int) * 2 * P, hipMemcpyDeviceToHost, st), id);
@@ -267,25 +311,44 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
     factored = true;
 }
 
+// The solve's fixed sequence on the internal plane-ordered vectors (bb -> xx):
+//   forward:  yy = blockdiag(LT) bb;  yy_p -= Mf_p yy_{p-1}  (p = 1 .. P-1, in order)
+//   backward: xx = blockdiag(UT) yy;  xx_p -= Nb_p xx_{p+1}  (p = P-2 .. 0, in order)
+void CoarseSolver::enqueue_core(hipStream_t s) const {
+    const char *id = "tg_solver:coarse";
+    const unsigned gr = grid_for(ntot, 4);
+    k_tri_rows<<<gr, 256, 0, s>>>(LT, d_adOff, d_off, rowPlane, ntot, 0, bb, yy);
+    for (int p = 1; p < P; ++p) {
+        if (nd[p] == 0 || nd[p - 1] == 0) continue;
+        launch_chain(Mf.p + mOff[p], nd[p], nd[p - 1], yy.p + off[p], yy.p + off[p - 1], yy.p + off[p], s);
+    }
+    k_tri_rows<<<gr, 256, 0, s>>>(Linv, d_adOff, d_off, rowPlane, ntot, 1, yy, xx);
+    for (int p = P - 2; p >= 0; --p) {
+        if (nd[p] == 0 || nd[p + 1] == 0) continue;
+        launch_chain(Hm.p + hOff[p], nd[p], nd[p + 1], xx.p + off[p], xx.p + off[p + 1], xx.p + off[p], s);
+    }
+    TG_LAUNCH_CHECK(id);
+}
+
 template <typename T>
 void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st) {
     const char *id = "tg_solver:coarse";
     if (!factored) fail(TG_ERR_STATE, id, "coarsest level not factorised (call assemble first)");
     k_pack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(r, bb, nodeMask, lidx, d_off, g, axis);
     TG_LAUNCH_CHECK(id);
-    for (int p = 0; p < P; ++p) {
-        const int64_t prev = p > 0 ? nd[p - 1] : 0;
-        if (nd[p] == 0) continue;
-        k_gemv_rows<<<grid_for(nd[p], 4), 256, 0, st>>>(W.p + wOff[p], nd[p], nd[p], prev, bb.p + off[p],
-                                                         prev ? yy.p + off[p - 1] : yy.p, yy.p + off[p]);
+    if (!exec) {
+        // the sequence reads only buffers owned here, so one capture serves every later solve
+        hipStream_t cap;
+        TG_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), id);
+        TG_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed), id);
+        enqueue_core(cap);
+        hipGraph_t graph;
+        TG_HIP(hipStreamEndCapture(cap, &graph), id);
+        TG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), id);
+        TG_HIP(hipGraphDestroy(graph), id);
+        TG_HIP(hipStreamDestroy(cap), id);
     }
-    for (int p = P - 1; p >= 0; --p) {
-        const int64_t next = p + 1 < P ? nd[p + 1] : 0;
-        if (nd[p] == 0) continue;
-        k_gemv_rows<<<grid_for(nd[p], 4), 256, 0, st>>>(V.p + vOff[p], nd[p], nd[p], next, yy.p + off[p],
-                                                         next ? xx.p + off[p + 1] : xx.p, xx.p + off[p]);
-    }
-    TG_LAUNCH_CHECK(id);
+    TG_HIP(hipGraphLaunch(exec, st), id);
     k_unpack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(xx, x, nodeMask, lidx, d_off, g, axis);
     TG_LAUNCH_CHECK(id);
 }

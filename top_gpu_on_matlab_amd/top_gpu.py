@@ -128,7 +128,7 @@ def subvoxel_energy(solver, U):
 
 def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, mixed_Precision=0,
             super_element=0, *, Emin=1e-6, E0=1.0, penal=3.0, nu=0.3, tol=1e-3, maxIT=800, x0=None,
-            out_path=None, verbose=False, model=None):
+            out_path=None, verbose=False, model=None, tuning=None):
     """TOP_GPU(inputModel, 'optCase', ..., 'V0', ..., 'rMin', ..., 'nLoop', ..., 'ft', ...,
     'mixed_Precision', ..., 'super_element', ...)  (TOP_GPU.m:1-292).
 
@@ -153,6 +153,8 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
     if se and ft != 2:
         raise ValueError('super-element mode requires ft = 2 (README: ft=2 is required)')
     solver = Solver(model)
+    if tuning:
+        solver.set_tuning(**tuning)
     Ne = model.num_elements
     dev = 'cuda'
     if se:

@@ -28,6 +28,10 @@ import numpy as np
 METRIC = 'PCG solves/sec + K·u HBM GB/s, mixed vs FP64 at 1e-6 rel compliance err'
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the FP64 finest K*u at 128x256x128, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a plain device copy of known size; the
+# runs and the factors are recorded in profiles/kfine_walsh_fp64_c5_128x256x128_pmc_traffic.md
+KFINE64_TRAFFIC = None
 
 
 def log(*a):
@@ -157,6 +161,9 @@ def main():
     log('setup + solid solves %.1fs (Ne=%d)' % (time.perf_counter() - t, run.Ne))
     hist = []
     for k in range(args.warmup + args.steps):
+        if k == args.warmup:  # per-launch HIP-event timing of the finest K*u inside the timed steps
+            for sv in run.solvers.values():
+                sv.time_kfine(True)
         r = run.iterate()
         hist.append(r)
         log('it %d: fp64 PCG %.1f ms (%d its) c=%.12e | mixed PCG %.1f ms (%d its) c=%.12e | rel %.1e | '
@@ -173,16 +180,24 @@ def main():
                       compliance=[h[p]['c'] for h in timed])
     res['fp64']['iteration_ms'] = 1e3 * np.mean([h['iter_s'] for h in timed])
     errs = [abs(h['mixed']['c'] - h['fp64']['c']) / abs(h['fp64']['c']) for h in timed]
-    # K*u kernel durations on the final state (HIP events on the solver stream)
-    s64, smx = run.solvers['fp64'], run.solvers['mixed']
-    ms64 = s64.time_kbyu(1, 'fp64', 50)
-    ms32 = smx.time_kbyu(1, 'fp32', 50)
+    # finest K*u: average launch duration over the timed steps (HIP events around every launch on the
+    # solver stream); FP64 = the PCG operator of the FP64 solver, FP32 = the mixed V-cycle's residual
+    # and smoothing applies
     Ne, Nn = run.Ne, run.model.num_nodes
+    kst = {}
+    for name, p, prec, epi in (('fp64_ku', 'fp64', 'fp64', 'ku'), ('fp64_resid_jac0', 'fp64', 'fp64', 'resid_jac0'),
+                               ('fp64_jacobi', 'fp64', 'fp64', 'jacobi'), ('fp32_resid_jac0', 'mixed', 'fp32', 'resid_jac0'),
+                               ('fp32_jacobi', 'mixed', 'fp32', 'jacobi')):
+        n, ms = run.solvers[p].kfine_timing(prec, epi)
+        kst[name] = dict(launches=n, avg_us=1e3 * ms / max(n, 1))
     b64, b32 = kbyu_alg_bytes(Nn, Ne, 8, 8, 8), kbyu_alg_bytes(Nn, Ne, 4, 4, 4)
-    gbs64, gbs32 = b64 / ms64 / 1e6, b32 / ms32 / 1e6
-    tf64 = 1152 * Ne / ms64 / 1e9
-    log('K*u fp64 %.1f us (%.0f GB/s alg, %.1f TFLOP/s); fp32 %.1f us (%.0f GB/s alg)'
-        % (ms64 * 1e3, gbs64, tf64, ms32 * 1e3, gbs32))
+    us64 = kst['fp64_ku']['avg_us']
+    gbs64 = b64 / us64 / 1e3
+    tf64 = 1152 * Ne / us64 / 1e6
+    for k, v in kst.items():
+        v['GBs_alg'] = (b64 if k.startswith('fp64') else b32) / v['avg_us'] / 1e3
+    log('finest K*u in the timed steps: ' + ', '.join('%s %.1f us x%d (%.0f GB/s alg)'
+                                                      % (k, v['avg_us'], v['launches'], v['GBs_alg']) for k, v in kst.items()))
     cpu = None
     if not args.no_cpu:
         try:
@@ -213,11 +228,11 @@ def main():
         'mixed': res['mixed'],
         'mixed_over_fp64': res['mixed']['solves_per_s'] / res['fp64']['solves_per_s'],
         'rel_compliance_err_mixed_vs_fp64': {'max': max(errs), 'per_iteration': errs},
-        'kbyu': {'fp64_us': ms64 * 1e3, 'fp64_GBs_alg': gbs64, 'fp64_TFLOPs': tf64,
-                 'fp32_us': ms32 * 1e3, 'fp32_GBs_alg': gbs32},
-        'roofline': {'kernel': 'finest-level K*u, FP64 (k_kfine<double>)', 'bound': 'hbm', 'achieved': gbs64,
-                     'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': gbs64 / HBM_PEAK_GBS, 'traffic': None,
-                     'fp64_flop_frac': tf64 / FP64_PEAK_TFLOPS},
+        'kbyu': kst,
+        'roofline': {'kernel': 'finest-level K*u, FP64, PCG operator (k_kfine_walsh<double,0,true>)', 'bound': 'hbm',
+                     'achieved': gbs64, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': gbs64 / HBM_PEAK_GBS,
+                     'traffic': KFINE64_TRAFFIC, 'bytes_alg': b64, 'avg_launch_us': us64, 'launches': kst['fp64_ku']['launches'],
+                     'dense_fp64_flop_frac': tf64 / FP64_PEAK_TFLOPS},
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)

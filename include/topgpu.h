@@ -216,9 +216,17 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
 /* Launch tuning of the finest-level standard-voxel K*u (no reference counterpart: the reference's
  * K*u is a MATLAB pagemtimes, TOP_GPU.m:1729-1741).  Keys:
  *   TG_TUNE_KFINE_KERNEL  0 = element-centric LDS-tiled z-march (default), 1 = node-centric gather
- *   TG_TUNE_KFINE_ZCHUNK  node planes per workgroup of the tiled kernel (1..4096, default 16) */
-enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2 };
+ *   TG_TUNE_KFINE_ZCHUNK  node planes per workgroup of the tiled kernel (1..4096, default 16)
+ *   TG_TUNE_TIME_KFINE    1 = bracket every tiled finest K*u launch with HIP events (and clear the
+ *                         record), 0 = stop
+ *   TG_TUNE_KELEM_PARTS   threads per element of the per-element-matrix K*u (coarse levels and the
+ *                         super-element finest level): 4, 6, 8, 12 or 24 (default 4) */
+enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4 };
 int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value);
+/* Launch count and summed duration (ms) of the timed finest K*u launches since timing was enabled,
+ * for precision 0 (FP64) / 1 (FP32) and epilogue 0 (K*u), 1 (r - K*u), 2 (damped Jacobi),
+ * 3 (r - K*x with the pre-smoothing x = omega r ./ diag fused in). */
+int tg_solver_kfine_timing(tg_solver *s, int32_t precision, int32_t epilogue, int64_t *launches, double *ms_total);
 
 /* ---------------------------------------------------------------------------------------
  * Layer 2: objective, sensitivities, filters and the OC update (device arrays in the
@@ -261,6 +269,18 @@ void tg_filter_destroy(tg_filter *f);
 /* tar = A K_F^{-1} A' src: element->node spread, Jacobi-PCG to `tol`, node->element average.
  * The reference calls it with tol = tol_/10 and maxIT = maxIT_ (TOP_GPU.m:1001).  Synchronises. */
 int tg_filter_apply(tg_filter *f, const double *d_src, double *d_tar, double tol, int32_t max_it, int32_t *its);
+
+/* Compliance and its design gradient, standard voxel mode (the sensitivity of TOP_GPU.m:158-174,
+ * 189-209 chained through the PDE filter, :214-218, without the 1/cSolid normalisation):
+ *   xPhys = filter(x) (f may be NULL: xPhys = x),  E = Emin + xPhys^p (E0 - Emin),
+ *   K(E) U = F (load case 1) by the MG-PCG (warm start from d_U when warm != 0; U returned in d_U,
+ *   compressed DOF order),  c = F'U = sum_e E_e u_e'K0 u_e,
+ *   dc/dxPhys_e = -p (E0 - Emin) xPhys_e^(p-1) u_e'K0 u_e,  d_dcdx = filter^T(dc/dxPhys) = filter(...).
+ * Compliance is self-adjoint (adjoint state -U), so the gradient reuses the forward solve.
+ * d_dcdx may be NULL (compliance only).  Synchronises. */
+int tg_compliance_grad(tg_solver *s, tg_filter *f, const double *d_x, double Emin, double E0, double penal,
+                       double tol, int32_t max_it, double filter_tol, int32_t warm, double *d_U, double *c_out,
+                       double *d_dcdx, int32_t *its_out);
 
 /* Super-element density filter: imfilter with the cone max(0, r - |d|) on the num-times refined box,
  * zero boundary (TOP_GPU.m:112-121, 220-227, 262-265); arrays are [num^3 x Ne] element-major. */

@@ -87,8 +87,11 @@ SIGNATURES = {
     'tg_solver_last_timing': (ctypes.c_int, [_VP, ctypes.POINTER(_D)]),
     'tg_solver_time_kbyu': (ctypes.c_int, [_VP, _I32, _I32, _I32, ctypes.POINTER(_D)]),
     'tg_solver_set_tuning': (ctypes.c_int, [_VP, _I32, _I32]),
+    'tg_solver_kfine_timing': (ctypes.c_int, [_VP, _I32, _I32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),
     'tg_solver_unit_compliance': (ctypes.c_int, [_VP, _VP, _VP]),
     'tg_solver_subvoxel_energy': (ctypes.c_int, [_VP, _VP, _VP]),
+    'tg_compliance_grad': (ctypes.c_int, [_VP, _VP, _VP, _D, _D, _D, _D, _I32, _D, _I32, _VP, ctypes.POINTER(_D), _VP,
+                                           ctypes.POINTER(_I32)]),
     'tg_simp_modulus': (ctypes.c_int, [_VP, _I64, _D, _D, _D, _VP, _VP]),
     'tg_sensitivity': (ctypes.c_int, [_VP, _VP, _I64, _I32, _D, _D, _D, _D, _VP, _VP]),
     'tg_elementwise': (ctypes.c_int, [_I32, _VP, _VP, _D, _I64, _VP, _VP]),
@@ -103,7 +106,7 @@ SIGNATURES = {
     'tg_convfilter_adjoint': (ctypes.c_int, [_VP, _VP, _VP]),
 }
 TG_EW_MUL, TG_EW_DIV_FLOOR = 0, 1
-TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK = 1, 2
+TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK, TG_TUNE_TIME_KFINE, TG_TUNE_KELEM_PARTS = 1, 2, 3, 4
 TG_RED_SUM, TG_RED_DOT, TG_RED_MAXABSDIFF, TG_RED_SHARPNESS = 0, 1, 2, 3
 
 _lib = None

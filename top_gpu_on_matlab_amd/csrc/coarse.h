@@ -1,14 +1,16 @@
 // Coarsest-level direct solver (replaces the host CHOLMOD factorisation and the per-V-cycle
 // host round trip of TOP_GPU.m:796-802, 1242-1256).
 //
-// The coarsest grid is cut into node planes along its longest axis.  With nodes ordered plane
-// by plane, K_c is block tridiagonal; its Cholesky factor is block bidiagonal:
-//   L_pp L_pp^T = A_pp - L_{p,p-1} L_{p,p-1}^T,   L_{p+1,p} = A_{p+1,p} L_pp^{-T}.
-// The factorisation runs on the device (rocSOLVER potrf/trtri, rocBLAS syrk/trsm/gemm).  A solve
-// is two batched triangular products (all planes at once) and two chains of P-1 dense GEMV steps:
-//   forward  y = blockdiag(inv(L_pp)) b,    y_p -= inv(L_pp) L_{p,p-1} y_{p-1}
-//   backward x = blockdiag(inv(L_pp)^T) y,  x_p -= inv(L_pp)^T L_{p+1,p}^T x_{p+1}
-// captured once as a HIP graph (its buffers are fixed at setup) and replayed on every solve.
+// The coarsest grid is cut into node planes along its longest axis, so K_c is block tridiagonal.
+// It is factorised from both ends towards a middle separator plane m ("twisted" block Cholesky):
+//   chain A, p = 0 .. m-1:     L_pp L_pp^T = A_pp - L_{p,p-1} L_{p,p-1}^T,  L_{p+1,p} = A_{p+1,p} L_pp^-T
+//   chain B, p = P-1 .. m+1:   L_pp L_pp^T = A_pp - L_{p,p+1} L_{p,p+1}^T,  L_{p-1,p} = A_{p-1,p} L_pp^-T
+//   separator:                 L_mm L_mm^T = A_mm - L_{m,m-1} L_{m,m-1}^T - L_{m,m+1} L_{m,m+1}^T
+// on the device (rocSOLVER potrf/trtri, rocBLAS syrk/trsm/gemm).  A solve is a batched triangular
+// product y = blockdiag(inv(L_pp)) b, the two forward chains advanced together (one launch per step,
+// ~P/2 steps) and the separator, then x = blockdiag(inv(L_pp)^T) y and the two backward chains
+// outwards from the separator; the whole sequence is captured once as a HIP graph (its buffers are
+// fixed at setup) and replayed on every solve.
 #pragma once
 
 #include <rocsolver/rocsolver.h>
@@ -24,13 +26,15 @@ struct CoarseSolver {
     std::vector<int64_t> nd;        // DOFs per plane
     std::vector<int64_t> off;       // DOF offset per plane
     std::vector<int64_t> adOff, asOff;  // offsets of the diagonal / sub-diagonal plane blocks
-    std::vector<int64_t> mOff, hOff;    // offsets of the padded forward / backward chain operators
+    int m = 0;                          // separator plane; chain A = [0, m), chain B = (m, P-1]
+    std::vector<int64_t> fOff, bOff;    // offsets of the padded forward / backward chain operators
+    std::vector<int64_t> fLd, bLd;      // their padded row lengths
     int64_t ntot = 0;
     Grid g{};
     DevBuf<int32_t> lidx;           // per full-grid node: local node index within its plane (-1 none)
     DevBuf<int64_t> d_off, d_adOff, d_asOff;
     DevBuf<int32_t> rowPlane;       // plane of each row of the stacked vector
-    DevBuf<double> Ad, As, Linv, LT, Mf, Hm;
+    DevBuf<double> Ad, As, Linv, LT, Fop, Bop;
     DevBuf<double> bb, yy, xx;
     DevBuf<rocblas_int> info;
     rocblas_handle h = nullptr;

@@ -98,51 +98,87 @@ __global__ __launch_bounds__(256) void k_tri_rows(const double *__restrict__ Tri
     if (lane == 0) out[R] = t;
 }
 
-// One step of a block recurrence, one wavefront per row:  y[r] = c[r] - sum_j M[r * ldm + j] v[j]
-// (M row-major n x m, rows zero-padded to ldm = nchunks * NK * 64; y may alias c).  The block
-// stages v in LDS chunk by chunk; each lane issues its NK matrix loads of a chunk before the
-// first use (no guards: the padding is zero), so a chunk costs one memory round trip.
+// One step of a block recurrence, one wavefront per row:  y[r] = c[r] - sum_j M[r * ld + j] v(j), with
+// v = [v1 (m1 entries); v2 (m - m1 entries)] and M row-major n x m, rows zero-padded to
+// ld = nchunks * NK * 64 (y may alias c).  Two independent problems share a launch (blocks
+// [0, nbA) take problem A, the rest problem B).  The block stages v in LDS chunk by chunk; each lane
+// issues its NK matrix loads of a chunk before the first use (no guards: the padding is zero), so a
+// chunk costs one memory round trip.
+struct ChainProb {
+    const double *M;
+    int64_t n, m, m1, ld;
+    int nchunks;
+    const double *c, *v1, *v2;
+    double *y;
+};
+
 template <int NK>
-__global__ __launch_bounds__(256) void k_chain_rows(const double *__restrict__ M, int64_t n, int64_t m, int64_t ldm,
-                                                    int nchunks, const double *c, const double *__restrict__ v,
-                                                    double *y) {
+__global__ __launch_bounds__(256) void k_chain_rows(const ChainProb A, const ChainProb B, unsigned nbA) {
     __shared__ double sv[NK * 64];
+    const bool isA = blockIdx.x < nbA;
+    const ChainProb &q = isA ? A : B;
     const int lane = threadIdx.x & 63;
-    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const double *row = M + (r < n ? r : 0) * ldm + lane;
+    const int64_t r = (int64_t)(isA ? blockIdx.x : blockIdx.x - nbA) * 4 + (threadIdx.x >> 6);
+    const double *row = q.M + (r < q.n ? r : 0) * q.ld + lane;
     double t = 0;
-    for (int q = 0; q < nchunks; ++q) {
+    for (int k0 = 0; k0 < q.nchunks; ++k0) {
         double a[NK];
 #pragma unroll
-        for (int k = 0; k < NK; ++k) a[k] = row[(int64_t)q * NK * 64 + 64 * k];
+        for (int k = 0; k < NK; ++k) a[k] = row[(int64_t)k0 * NK * 64 + 64 * k];
         __syncthreads();
         for (int j = threadIdx.x; j < NK * 64; j += 256) {
-            const int64_t jj = (int64_t)q * NK * 64 + j;
-            sv[j] = jj < m ? v[jj] : 0.0;
+            const int64_t jj = (int64_t)k0 * NK * 64 + j;
+            sv[j] = jj < q.m1 ? q.v1[jj] : (jj < q.m ? q.v2[jj - q.m1] : 0.0);
         }
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < NK; ++k) t += a[k] * sv[lane + 64 * k];
     }
     for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o, 64);
-    if (lane == 0 && r < n) y[r] = c[r] - t;
+    if (lane == 0 && r < q.n) q.y[r] = q.c[r] - t;
 }
 
 // Padded row length (multiple of 64, or of 1024 beyond 1024) and the chunking used for it.
 inline int64_t chain_ld(int64_t m) { return m <= 1024 ? ((m + 63) / 64) * 64 : ((m + 1023) / 1024) * 1024; }
+inline int chain_nk(int64_t ld) { return (int)(ld <= 1024 ? ld / 64 : 16); }
 
-void launch_chain(const double *M, int64_t n, int64_t m, const double *c, const double *v, double *y, hipStream_t s) {
-    const int64_t ld = chain_ld(m);
-    const unsigned gr = grid_for(n, 4);
-    const int nk = (int)(ld <= 1024 ? ld / 64 : 16), nch = (int)(ld <= 1024 ? 1 : ld / 1024);
+ChainProb chain_prob(const double *M, int64_t n, int64_t m, int64_t m1, int64_t ld, const double *c, const double *v1,
+                     const double *v2, double *y) {
+    ChainProb q;
+    q.M = M;
+    q.n = n;
+    q.m = m;
+    q.m1 = m1;
+    q.ld = ld;
+    q.nchunks = (int)(ld <= 1024 ? 1 : ld / 1024);
+    q.c = c;
+    q.v1 = v1;
+    q.v2 = v2;
+    q.y = y;
+    return q;
+}
+
+void launch_chain_nk(int nk, const ChainProb &A, const ChainProb &B, unsigned nbA, unsigned nbB, hipStream_t s) {
     switch (nk) {
 #define TG_CHAIN_CASE(K) \
-    case K: k_chain_rows<K><<<gr, 256, 0, s>>>(M, n, m, ld, nch, c, v, y); break;
+    case K: k_chain_rows<K><<<nbA + nbB, 256, 0, s>>>(A, B, nbA); break;
         TG_CHAIN_CASE(1) TG_CHAIN_CASE(2) TG_CHAIN_CASE(3) TG_CHAIN_CASE(4) TG_CHAIN_CASE(5) TG_CHAIN_CASE(6)
         TG_CHAIN_CASE(7) TG_CHAIN_CASE(8) TG_CHAIN_CASE(9) TG_CHAIN_CASE(10) TG_CHAIN_CASE(11) TG_CHAIN_CASE(12)
         TG_CHAIN_CASE(13) TG_CHAIN_CASE(14) TG_CHAIN_CASE(15) TG_CHAIN_CASE(16)
 #undef TG_CHAIN_CASE
         default: fail(TG_ERR_ARGS, "tg_solver:coarse", "bad chain chunking");
+    }
+}
+
+// One launch for both problems when they use the same chunking, otherwise one each (n == 0: absent).
+void launch_chain(const ChainProb &A, const ChainProb &B, hipStream_t s) {
+    const unsigned nbA = A.n ? grid_for(A.n, 4) : 0, nbB = B.n ? grid_for(B.n, 4) : 0;
+    const int nkA = A.n ? chain_nk(A.ld) : 0, nkB = B.n ? chain_nk(B.ld) : 0;
+    if (nbA && nbB && nkA == nkB) {
+        launch_chain_nk(nkA, A, B, nbA, nbB, s);
+    } else {
+        if (nbA) launch_chain_nk(nkA, A, A, nbA, 0, s);
+        if (nbB) launch_chain_nk(nkB, B, B, nbB, 0, s);
     }
 }
 
@@ -217,16 +253,25 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
         off[p + 1] = off[p] + nd[p];
     }
     ntot = off[P];
+    m = P / 2;
     adOff.assign(P + 1, 0);
     asOff.assign(P + 1, 0);
-    mOff.assign(P + 1, 0);
-    hOff.assign(P + 1, 0);
+    fOff.assign(P + 1, 0);
+    bOff.assign(P + 1, 0);
+    fLd.assign(P, 0);
+    bLd.assign(P, 0);
     for (int p = 0; p < P; ++p) {
         const int64_t prev = p > 0 ? nd[p - 1] : 0, next = p + 1 < P ? nd[p + 1] : 0;
         adOff[p + 1] = adOff[p] + nd[p] * nd[p];
         asOff[p + 1] = asOff[p] + nd[p] * prev;
-        mOff[p + 1] = mOff[p] + nd[p] * (prev ? chain_ld(prev) : 0);
-        hOff[p + 1] = hOff[p] + nd[p] * (next ? chain_ld(next) : 0);
+        // forward operator columns: the plane it is eliminated against (both neighbours for m)
+        const int64_t fc = p < m ? prev : (p > m ? next : prev + next);
+        // backward operator columns: the plane it is back-substituted from (none for m)
+        const int64_t bc = p < m ? next : (p > m ? prev : 0);
+        fLd[p] = fc ? chain_ld(fc) : 0;
+        bLd[p] = bc ? chain_ld(bc) : 0;
+        fOff[p + 1] = fOff[p] + nd[p] * fLd[p];
+        bOff[p + 1] = bOff[p] + nd[p] * bLd[p];
     }
     std::vector<int32_t> rp(std::max<int64_t>(ntot, 1), 0);
     for (int p = 0; p < P; ++p)
@@ -240,10 +285,10 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
     Linv.alloc(adOff[P]);
     LT.alloc(adOff[P]);
     As.alloc(std::max<int64_t>(asOff[P], 1));
-    Mf.alloc(std::max<int64_t>(mOff[P], 1));
-    Hm.alloc(std::max<int64_t>(hOff[P], 1));
-    Mf.zero(st);  // row padding stays zero
-    Hm.zero(st);
+    Fop.alloc(std::max<int64_t>(fOff[P], 1));
+    Bop.alloc(std::max<int64_t>(bOff[P], 1));
+    Fop.zero(st);  // row padding stays zero
+    Bop.zero(st);
     bb.alloc(ntot);
     yy.alloc(ntot);
     xx.alloc(ntot);
@@ -264,41 +309,81 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
                                                           d_asOff, Ad, As);
     TG_LAUNCH_CHECK(id);
     const double one = 1.0, mone = -1.0, zero = 0.0;
-    for (int p = 0; p < P; ++p) {
+    // As_p (n_p x n_{p-1}, column-major) holds A_{p,p-1}.  Chain A turns As_{p+1} into
+    // L_{p+1,p} = A_{p+1,p} L_pp^-T (right trsm); chain B turns As_p into X_p = L_pp^-1 A_{p,p-1}
+    // = L_{p-1,p}^T (left trsm).
+    auto factor_plane = [&](int p) {
         const rocblas_int np = (rocblas_int)nd[p];
-        if (np == 0) continue;
         double *S = Ad.p + adOff[p];
-        if (p > 0 && nd[p - 1] > 0)
-            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, np, (rocblas_int)nd[p - 1], &mone,
-                             As.p + asOff[p], np, &one, S, np), "rocblas_dsyrk");
         rb(rocsolver_dpotrf(h, rocblas_fill_lower, np, S, np, info.p + p), "rocsolver_dpotrf");
         k_lower_only<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(S, Linv.p + adOff[p], np);
         TG_LAUNCH_CHECK(id);
         rb(rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, np, Linv.p + adOff[p], np, info.p + P + p),
            "rocsolver_dtrtri");
-        if (p + 1 < P && nd[p + 1] > 0)
+    };
+    for (int p = 0; p < m; ++p) {  // chain A
+        const rocblas_int np = (rocblas_int)nd[p];
+        if (np == 0) continue;
+        if (p > 0 && nd[p - 1] > 0)
+            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, np, (rocblas_int)nd[p - 1], &mone,
+                             As.p + asOff[p], np, &one, Ad.p + adOff[p], np), "rocblas_dsyrk");
+        factor_plane(p);
+        if (nd[p + 1] > 0)
             rb(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                             rocblas_diagonal_non_unit, (rocblas_int)nd[p + 1], np, &one, S, np, As.p + asOff[p + 1],
-                             (rocblas_int)nd[p + 1]), "rocblas_dtrsm");
+                             rocblas_diagonal_non_unit, (rocblas_int)nd[p + 1], np, &one, Ad.p + adOff[p], np,
+                             As.p + asOff[p + 1], (rocblas_int)nd[p + 1]), "rocblas_dtrsm A");
     }
-    // Solve operators (all row-major): LT_p = inv(L_pp) and UT_p = inv(L_pp)^T (the column-major
-    // Linv itself), Mf_p = inv(L_pp) L_{p,p-1} (n_p x n_{p-1}, from (As_p^T Linv_p^T) column-major)
-    // and Hm_p = inv(L_pp)^T L_{p+1,p}^T (n_p x n_{p+1}, from L_{p+1,p} inv(L_pp) column-major), the
-    // last two with rows zero-padded to chain_ld.
+    for (int p = P - 1; p > m; --p) {  // chain B
+        const rocblas_int np = (rocblas_int)nd[p];
+        if (np == 0) continue;
+        if (p + 1 < P && nd[p + 1] > 0)  // S_p -= X_{p+1}^T X_{p+1}
+            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, np, (rocblas_int)nd[p + 1], &mone,
+                             As.p + asOff[p + 1], (rocblas_int)nd[p + 1], &one, Ad.p + adOff[p], np), "rocblas_dsyrk");
+        factor_plane(p);
+        if (nd[p - 1] > 0)
+            rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                             rocblas_diagonal_non_unit, np, (rocblas_int)nd[p - 1], &one, Ad.p + adOff[p], np,
+                             As.p + asOff[p], np), "rocblas_dtrsm B");
+    }
+    if (nd[m] > 0) {  // separator
+        const rocblas_int nm = (rocblas_int)nd[m];
+        if (m > 0 && nd[m - 1] > 0)
+            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, nm, (rocblas_int)nd[m - 1], &mone,
+                             As.p + asOff[m], nm, &one, Ad.p + adOff[m], nm), "rocblas_dsyrk");
+        if (m + 1 < P && nd[m + 1] > 0)
+            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, nm, (rocblas_int)nd[m + 1], &mone,
+                             As.p + asOff[m + 1], (rocblas_int)nd[m + 1], &one, Ad.p + adOff[m], nm), "rocblas_dsyrk");
+        factor_plane(m);
+    }
+    // Solve operators, row-major with zero-padded rows (column-major results of transposed products):
+    //   LT_p = inv(L_pp) (row-major copy); inv(L_pp)^T is the column-major Linv itself.
+    //   forward  A: inv(L_pp) L_{p,p-1}       = (As_p^T Linv_p^T)^T      (n_p x n_{p-1})
+    //            B: inv(L_pp) L_{p,p+1}       = (X_{p+1} Linv_p^T)^T     (n_p x n_{p+1})
+    //            m: [inv(L_mm) L_{m,m-1} | inv(L_mm) L_{m,m+1}]
+    //   backward A: inv(L_pp)^T L_{p+1,p}^T   = (As_{p+1} Linv_p)^T      (n_p x n_{p+1})
+    //            B: inv(L_pp)^T L_{p-1,p}^T   = (X_p^T Linv_p)^T         (n_p x n_{p-1})
     for (int p = 0; p < P; ++p) {
         const rocblas_int np = (rocblas_int)nd[p];
         if (np == 0) continue;
         const int64_t prev = p > 0 ? nd[p - 1] : 0, next = p + 1 < P ? nd[p + 1] : 0;
-        k_transpose_block<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Linv.p + adOff[p], LT.p + adOff[p], np);
+        const double *Lp = Linv.p + adOff[p];
+        k_transpose_block<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Lp, LT.p + adOff[p], np);
         TG_LAUNCH_CHECK(id);
-        if (prev > 0)
+        double *F = Fop.p + fOff[p], *B = Bop.p + bOff[p];
+        const rocblas_int fl = (rocblas_int)fLd[p], bl = (rocblas_int)bLd[p];
+        if (p <= m && prev > 0)  // A-side coupling (chain A, or the separator's first block)
             rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_transpose, (rocblas_int)prev, np, np,
-                             &one, As.p + asOff[p], np, Linv.p + adOff[p], np, &zero, Mf.p + mOff[p],
-                             (rocblas_int)chain_ld(prev)), "rocblas_dgemm Mf");
-        if (next > 0)
+                             &one, As.p + asOff[p], np, Lp, np, &zero, F, fl), "rocblas_dgemm F A");
+        if (p >= m && next > 0)  // B-side coupling (chain B, or the separator's second block)
+            rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)next, np, np,
+                             &one, As.p + asOff[p + 1], (rocblas_int)next, Lp, np, &zero, F + (p == m ? prev : 0),
+                             fl), "rocblas_dgemm F B");
+        if (p < m && next > 0)
             rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)next, np, np, &one,
-                             As.p + asOff[p + 1], (rocblas_int)next, Linv.p + adOff[p], np, &zero,
-                             Hm.p + hOff[p], (rocblas_int)chain_ld(next)), "rocblas_dgemm Hm");
+                             As.p + asOff[p + 1], (rocblas_int)next, Lp, np, &zero, B, bl), "rocblas_dgemm B A");
+        if (p > m && prev > 0)
+            rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)prev, np, np, &one,
+                             As.p + asOff[p], np, Lp, np, &zero, B, bl), "rocblas_dgemm B B");
     }
     std::vector<rocblas_int> hinfo(2 * P);
     TG_HIP(hipMemcpyAsync(hinfo.data(), info.p, sizeof(rocblas_int) * 2 * P, hipMemcpyDeviceToHost, st), id);
@@ -312,21 +397,36 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
 }
 
 // The solve's fixed sequence on the internal plane-ordered vectors (bb -> xx):
-//   forward:  yy = blockdiag(LT) bb;  yy_p -= Mf_p yy_{p-1}  (p = 1 .. P-1, in order)
-//   backward: xx = blockdiag(UT) yy;  xx_p -= Nb_p xx_{p+1}  (p = P-2 .. 0, in order)
+//   yy = blockdiag(LT) bb; forward chains A (p = 1 .. m-1) and B (p = P-2 .. m+1) step by step together;
+//   the separator; xx = blockdiag(Linv^T) yy; backward chains A (p = m-1 .. 0) and B (p = m+1 .. P-1).
 void CoarseSolver::enqueue_core(hipStream_t s) const {
     const char *id = "tg_solver:coarse";
     const unsigned gr = grid_for(ntot, 4);
+    const ChainProb none = chain_prob(nullptr, 0, 0, 0, 0, nullptr, nullptr, nullptr, nullptr);
+    auto fwd = [&](int p, int q) -> ChainProb {  // y_p -= Fop_p y_q
+        if (p < 0 || p >= P || nd[p] == 0 || q < 0 || q >= P || nd[q] == 0) return none;
+        return chain_prob(Fop.p + fOff[p], nd[p], nd[q], nd[q], fLd[p], yy.p + off[p], yy.p + off[q], nullptr,
+                          yy.p + off[p]);
+    };
+    auto bwd = [&](int p, int q) -> ChainProb {  // x_p -= Bop_p x_q
+        if (p < 0 || p >= P || nd[p] == 0 || q < 0 || q >= P || nd[q] == 0) return none;
+        return chain_prob(Bop.p + bOff[p], nd[p], nd[q], nd[q], bLd[p], xx.p + off[p], xx.p + off[q], nullptr,
+                          xx.p + off[p]);
+    };
     k_tri_rows<<<gr, 256, 0, s>>>(LT, d_adOff, d_off, rowPlane, ntot, 0, bb, yy);
-    for (int p = 1; p < P; ++p) {
-        if (nd[p] == 0 || nd[p - 1] == 0) continue;
-        launch_chain(Mf.p + mOff[p], nd[p], nd[p - 1], yy.p + off[p], yy.p + off[p - 1], yy.p + off[p], s);
+    const int lenA = m, lenB = P - 1 - m;  // planes per chain
+    for (int k = 1; k < std::max(lenA, lenB); ++k)
+        launch_chain(k < lenA ? fwd(k, k - 1) : none, k < lenB ? fwd(P - 1 - k, P - k) : none, s);
+    if (nd[m] > 0) {  // separator: y_m -= Fop_m [y_{m-1}; y_{m+1}]
+        const int64_t na = (m > 0) ? nd[m - 1] : 0, nb = (m + 1 < P) ? nd[m + 1] : 0;
+        if (na + nb > 0)
+            launch_chain(chain_prob(Fop.p + fOff[m], nd[m], na + nb, na, fLd[m], yy.p + off[m],
+                                    na ? yy.p + off[m - 1] : yy.p, nb ? yy.p + off[m + 1] : yy.p, yy.p + off[m]),
+                         none, s);
     }
     k_tri_rows<<<gr, 256, 0, s>>>(Linv, d_adOff, d_off, rowPlane, ntot, 1, yy, xx);
-    for (int p = P - 2; p >= 0; --p) {
-        if (nd[p] == 0 || nd[p + 1] == 0) continue;
-        launch_chain(Hm.p + hOff[p], nd[p], nd[p + 1], xx.p + off[p], xx.p + off[p + 1], xx.p + off[p], s);
-    }
+    for (int k = 1; k <= std::max(lenA, lenB); ++k)
+        launch_chain(k <= lenA ? bwd(m - k, m - k + 1) : none, k <= lenB ? bwd(m + k, m + k - 1) : none, s);
     TG_LAUNCH_CHECK(id);
 }
 

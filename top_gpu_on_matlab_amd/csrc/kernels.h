@@ -34,7 +34,7 @@ struct Grid {
     __host__ __device__ int64_t elems() const { return (int64_t)EY * EX * EZ; }
 };
 
-enum { EPI_KU = 0, EPI_RESID = 1, EPI_JACOBI = 2 };
+enum { EPI_KU = 0, EPI_RESID = 1, EPI_JACOBI = 2, EPI_RESID_JAC0 = 3 };
 
 // Load the 27 neighbour nodes' values (fixed DOFs zeroed when MASKED).
 template <typename T, bool MASKED>
@@ -162,16 +162,16 @@ __global__ __launch_bounds__(256) void k_kfine(const T *__restrict__ u, T *__res
 // levels (Galerkin Ks, MASKED = false) and the super-element finest level (K_mat = I_mat * E_sub
 // with BC masks, MASKED = true); y_e = u_e^T * K_e (TOP_GPU.m:757-760).
 // Two stages.  Stage 1: y_e = u_e^T K_e for every solid
-// element, four threads per element (6 output columns each; consecutive threads take consecutive
-// elements, so every matrix entry row is streamed coalesced), written entry-major
+// element, PARTS threads per element (24 / PARTS output columns each; consecutive threads take
+// consecutive elements, so every matrix entry row is streamed coalesced), written entry-major
 // (f[j * ldk + e]).  Stage 2: each node sums its (up to 8) element contributions in the fixed
 // element order (z, x, y), then applies identity rows for fixed DOFs (MASKED) and the epilogue.
-constexpr int KE_PARTS = 4;
-template <typename T, bool MASKED>
+template <typename T, bool MASKED, int PARTS>
 __global__ __launch_bounds__(256) void k_kelem_vec(const T *__restrict__ u, T *__restrict__ f,
                                                    const T *__restrict__ Km, int64_t ldk,
                                                    const uint8_t *__restrict__ eleMask,
                                                    const uint8_t *__restrict__ mask, const Grid g) {
+    constexpr int KE_PARTS = PARTS;
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int part = (int)(gid / ldk);
     const int64_t e = gid - (int64_t)part * ldk;
@@ -302,6 +302,90 @@ __global__ __launch_bounds__(256) void k_restrict(const T *__restrict__ rf, T *_
         rc[3 * J] = a0;
         rc[3 * J + 1] = a1;
         rc[3 * J + 2] = a2;
+    }
+}
+
+// Restriction, element-centric and LDS-tiled (same sums as k_restrict, in separable order).  A
+// workgroup owns RT_E x RT_E coarse elements (y, x) over RT_Z coarse layers and marches through the
+// fine node planes of that box: each plane of v = r_f ./ coeff is staged in LDS (the next one is
+// prefetched into registers), every lane forms, for its (element, vertex column (jy, jx), component),
+// the 2D trilinear sum over the element's (S+1)^2 fine nodes of the plane, and folds it into the
+// vertex's two z weights.  When a coarse layer is complete the lane writes its part of the element's
+// 8-vertex contributions C_e (entry-major f[(3 j + c) * ldk + e]); k_kelem_gather then sums them per
+// coarse node over the non-empty elements, which is r_c = P^T r_f.
+constexpr int RT_E = 8, RT_Z = 2;
+constexpr int RT_THREADS = RT_E * RT_E * 12;  // one lane per (element, vertex column, component)
+template <typename T, int S>
+__global__ __launch_bounds__(RT_THREADS) void k_restrict_tiled(const T *__restrict__ rf, const T *__restrict__ coeff,
+                                                               const Grid gc, const Grid gf, T *__restrict__ f,
+                                                               int64_t ldk) {
+    constexpr int FN = RT_E * S + 1;                      // fine nodes per tile side
+    constexpr int NV = 3 * FN * FN;                       // staged values per plane
+    constexpr int NI = (NV + RT_THREADS - 1) / RT_THREADS;  // staged values per lane
+    __shared__ T sV[NV];
+    const int t = threadIdx.x;
+    const int tilesY = (gc.EY + RT_E - 1) / RT_E, tilesX = (gc.EX + RT_E - 1) / RT_E;
+    int bid = blockIdx.x;
+    const int tz = bid / (tilesY * tilesX);
+    bid -= tz * tilesY * tilesX;
+    const int e0y = (bid % tilesY) * RT_E, e0x = (bid / tilesY) * RT_E, e0z = tz * RT_Z;
+    const int ez1 = min(e0z + RT_Z, gc.EZ);
+    const int64_t NYX = (int64_t)gf.NY * gf.NX;
+    // this lane's output: component, vertex column (jy, jx), element (ly, lx) of the tile
+    const int oc = t % 3, q = t / 3, jy = q & 1, jx = (q >> 1) & 1, el = q >> 2;
+    const int ly = el % RT_E, lx = el / RT_E;
+    const int ey = e0y + ly, ex = e0x + lx;
+    const bool own = ey < gc.EY && ex < gc.EX;
+    T pre[NI];
+    auto fetch = [&](int iz) {
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int i = min(t + RT_THREADS * k, NV - 1);
+            const int node = i / 3, c = i - 3 * node;
+            const int gy = S * e0y + node % FN, gx = S * e0x + node / FN;
+            const bool ok = gy < gf.NY && gx < gf.NX && iz < gf.NZ;
+            const int64_t n = ok ? gy + (int64_t)gf.NY * gx + NYX * iz : 0;
+            const T v = rf[3 * n + c] / coeff[n];
+            pre[k] = ok ? v : (T)0;
+        }
+    };
+    const T *pv = sV + oc * FN * FN + S * ly + FN * S * lx;
+    T acc0 = 0, acc1 = 0;
+    const int iz0 = S * e0z, iz1 = S * ez1;
+    fetch(iz0);
+    for (int iz = iz0; iz <= iz1; ++iz) {
+        lds_barrier();
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const int i = t + RT_THREADS * k;
+            if (i < NV) sV[(i % 3) * FN * FN + i / 3] = pre[k];
+        }
+        lds_barrier();
+        if (iz < iz1) fetch(iz + 1);
+        T val = 0;
+#pragma unroll
+        for (int b = 0; b <= S; ++b) {
+            T sy = 0;
+#pragma unroll
+            for (int a = 0; a <= S; ++a) sy += (jy ? (T)a / S : (T)(S - a) / S) * pv[a + FN * b];
+            val += (jx ? (T)b / S : (T)(S - b) / S) * sy;
+        }
+        const int cz = (iz - iz0) % S;  // offset inside the current coarse layer
+        if (cz == 0 && iz > iz0) {
+            // this plane closes the layer below (top face, z weight 1) and opens the next one
+            acc1 += val;
+            if (own) {
+                const int64_t e = ey + (int64_t)gc.EY * (ex + (int64_t)gc.EX * (e0z + (iz - iz0) / S - 1));
+                f[(int64_t)(3 * vtx_of(jy, jx, 0) + oc) * ldk + e] = acc0;
+                f[(int64_t)(3 * vtx_of(jy, jx, 1) + oc) * ldk + e] = acc1;
+            }
+            acc0 = val;
+            acc1 = 0;
+        } else {
+            const T tz = (T)cz / S;
+            acc0 += ((T)1 - tz) * val;
+            acc1 += tz * val;
+        }
     }
 }
 

@@ -132,11 +132,17 @@ __device__ __forceinline__ void walsh_inv(T x[8]) {
 
 // CK: KW is the compile-time unit matrix for nu = 0.3 and each element is scaled by cs * E_e;
 // otherwise KW (cellSize included) comes from kw and cs is ignored.
-template <typename T, int EPI, bool CK>
+// DOT: also write this workgroup's share of a CG dot product to partial[blockIdx.x] (FP64, fixed
+// order): u.out for the plain product (p'Kp), r.out for the Jacobi epilogue (z'r).
+// EPI_RESID_JAC0: the V-cycle's pre-smoothing fused in: the operand is x = omega r ./ diag (formed
+// while staging; u is not read), x is written to xout and out = r - K x.
+template <typename T, int EPI, bool CK, bool DOT>
 __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) void k_kfine_walsh(const T *__restrict__ u, T *__restrict__ out,
                                                      const T *__restrict__ E, const uint8_t *__restrict__ mask,
                                                      const KWVals<T> kw, T cs, const Grid g, const KTiles kt,
-                                                     const T *__restrict__ r, const T *__restrict__ diag, T omega) {
+                                                     const T *__restrict__ r, const T *__restrict__ diag, T omega,
+                                                     double *__restrict__ partial, T *__restrict__ xout) {
+    constexpr bool J0 = EPI == EPI_RESID_JAC0;
     __shared__ T sU[2][3][KT_PN];
     __shared__ uint8_t sM[2][KT_PN];
     // FP64 element vectors pass through LDS in two halves (lower, then upper vertices) so that four
@@ -166,7 +172,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
 
     // Prefetch: loads are unconditional (clamped addresses) and all masking is applied at commit
     // time, so no wait for the loaded values is needed until the next layer.
-    T pre[KT_NI];
+    T pre[KT_NI], preD[J0 ? KT_NI : 1];
     uint8_t preM[KT_NI];
     bool preOk[KT_NI];
     auto fetch = [&](int z) {
@@ -178,7 +184,12 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
             const int gy = y0 - 1 + py, gx = x0 - 1 + px;
             const bool ok = (unsigned)gy < (unsigned)g.NY && (unsigned)gx < (unsigned)g.NX && (unsigned)z < (unsigned)g.NZ;
             const int64_t n = ok ? gy + (int64_t)g.NY * gx + NYX * z : 0;
-            pre[k] = u[3 * n + c];
+            if (J0) {
+                pre[k] = r[3 * n + c];
+                preD[J0 ? k : 0] = diag[3 * n + c];
+            } else {
+                pre[k] = u[3 * n + c];
+            }
             preM[k] = mask[n];
             preOk[k] = ok;
         }
@@ -190,7 +201,8 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
             if (i < 3 * KT_PN) {
                 const int node = i / 3, c = i - 3 * node;
                 const uint8_t mk = preOk[k] ? preM[k] : (uint8_t)0;
-                sU[slot][c][node] = (preOk[k] && !(mk & (1 << c))) ? pre[k] : (T)0;
+                const T v = J0 ? omega * (pre[k] / preD[J0 ? k : 0]) : pre[k];
+                sU[slot][c][node] = (preOk[k] && !(mk & (1 << c))) ? v : (T)0;
                 if (c == 0) sM[slot][node] = mk;
             }
         }
@@ -207,6 +219,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
     fetch(z0);
     fetchE(z0 - 1);
     T part[3] = {0, 0, 0};  // upper-vertex contributions to the owned node of the next plane
+    double dsum = 0;        // DOT: this lane's share, accumulated plane by plane
     int cur = 0;
     for (int ez = z0 - 1; ez < z1; ++ez) {
         commit(cur ^ 1);  // node plane ez + 1
@@ -276,15 +289,33 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
                 }
             if (ez >= z0) {
                 const int64_t n = (y0 + oy) + (int64_t)g.NY * (x0 + ox) + NYX * ez;
-                if (mk & NODE_ACTIVE) {
+                if (J0 && (mk & NODE_ACTIVE)) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const T rv = r[3 * n + d];
+                        const T xv = omega * (rv / diag[3 * n + d]);
+                        xout[3 * n + d] = xv;
+                        out[3 * n + d] = rv - ((mk & (1 << d)) ? xv : lo[d]);
+                    }
+                } else if (mk & NODE_ACTIVE) {
 #pragma unroll
                     for (int d = 0; d < 3; ++d)
                         if (mk & (1 << d)) lo[d] = u[3 * n + d];
                     epilogue<T, EPI>(out, n, lo, r, diag, u, omega);
+                    if (DOT) {
+                        const T *w = EPI == EPI_KU ? u : r;
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) dsum += (double)out[3 * n + d] * (double)w[3 * n + d];
+                    }
                 } else {
                     out[3 * n] = 0;
                     out[3 * n + 1] = 0;
                     out[3 * n + 2] = 0;
+                    if (J0) {
+                        xout[3 * n] = 0;
+                        xout[3 * n + 1] = 0;
+                        xout[3 * n + 2] = 0;
+                    }
                 }
             }
         }
@@ -309,6 +340,18 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
         part[1] = up[1];
         part[2] = up[2];
         cur ^= 1;
+    }
+    if (DOT) {
+        // fixed-order workgroup sum (sF is free: every lane has passed its last read of it)
+        lds_barrier();
+        double *red = reinterpret_cast<double *>(&sF[0][0]);
+        red[t] = dsum;
+        lds_barrier();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w) red[t] += red[t + w];
+            lds_barrier();
+        }
+        if (t == 0) partial[blockIdx.x] = red[0];
     }
 }
 

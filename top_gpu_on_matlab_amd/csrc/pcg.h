@@ -39,6 +39,22 @@ __global__ static void k_pcg_direction(double *__restrict__ p, const double *__r
     if (i < n) p[i] = z[i] + scal[ix2] / scal[ix1] * p[i];
 }
 
+// z64 = (double) z32 and partial sums of z64 . r64 (the mixed V-cycle's exit, fused with z'r)
+__global__ __launch_bounds__(RED_THREADS) static void k_convert_dot(const float *__restrict__ z32,
+                                                                    const double *__restrict__ r, double *__restrict__ z,
+                                                                    int64_t n, double *__restrict__ partial) {
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
+    double s = 0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const double v = (double)z32[i];
+        z[i] = v;
+        s += v * r[i];
+    }
+    s = block_sum<double>(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 __global__ static void k_sub(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c,
                              int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -77,36 +93,35 @@ struct PcgWork {
         return h_scal[slot];
     }
 
-    // atx(u, out): out = A u;  ptv(r, z): z = M r.  y holds the start vector (zeros unless warm).
+    // atx(u, out, slot): out = A u;  ptv(r, z, slot): z = M r.  Each returns true when it has also
+    // left u.out (resp. z.r) in scal[slot]; otherwise the dot is computed here.  y holds the start
+    // vector (zeros unless warm).
     template <class AtX, class PtV>
     void run(AtX atx, PtV ptv, const double *b, double *y, double tol, int maxIT, bool warm, int *its_out,
              double *res_out, const char *id) {
         dot(b, b, S_BB);
         const double normB = std::sqrt(read(S_BB));
         if (warm) {
-            atx(y, v.p);
+            atx(y, v.p, -1);
             k_sub<<<grid_for(n, 256), 256, 0, st>>>(b, v, r, n);
             TG_LAUNCH_CHECK(id);
         } else {
             TG_HIP(hipMemcpyAsync(r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, st), id);
         }
-        ptv(r.p, z.p);
-        TG_HIP(hipMemcpyAsync(p.p, z.p, n * sizeof(double), hipMemcpyDeviceToDevice, st), id);
         int sx1 = S_XA, sx2 = S_XB;
-        dot(z, r, sx1);
+        if (!ptv(r.p, z.p, sx1)) dot(z, r, sx1);
+        TG_HIP(hipMemcpyAsync(p.p, z.p, n * sizeof(double), hipMemcpyDeviceToDevice, st), id);
         int its = 0;
         double res = 0;
         for (its = 1; its <= maxIT; ++its) {
-            atx(p.p, v.p);
-            dot(p, v, S_PV);
+            if (!atx(p.p, v.p, S_PV)) dot(p, v, S_PV);
             k_pcg_update<<<RED_BLOCKS, RED_THREADS, 0, st>>>(y, r, p, v, n, scal, sx1, S_PV, partial);
             k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR);
             TG_LAUNCH_CHECK(id);
             res = std::sqrt(read(S_RR)) / normB;
             if (!std::isfinite(res)) fail(TG_ERR_NUMERIC, id, "non-finite residual at iteration %d", its);
             if (res < tol) break;
-            ptv(r.p, z.p);
-            dot(z, r, sx2);
+            if (!ptv(r.p, z.p, sx2)) dot(z, r, sx2);
             k_pcg_direction<<<grid_for(n, 256), 256, 0, st>>>(p, z, n, scal, sx2, sx1);
             TG_LAUNCH_CHECK(id);
             std::swap(sx1, sx2);

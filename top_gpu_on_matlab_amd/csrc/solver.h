@@ -39,7 +39,17 @@ struct Solver {
     bool walsh_ok = false;  // the finest element matrix has the isotropic Walsh-basis pattern
     KWVals<double> kw64{};
     KWVals<float> kw32{};
+    // optional per-launch timing of the finest-level K*u (HIP events on the solver stream)
+    struct TimedLaunch {
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        int key = 0;  // 4 * (FP32 ? 1 : 0) + epilogue
+    };
+    bool time_fine = false;
+    std::vector<TimedLaunch> tpool;
+    size_t tused = 0;
+    hipEvent_t *timing_slot(int key);
     int kfine_zchunk = 16;  // node planes per workgroup of the tiled kernel
+    int kelem_parts = 4;    // threads per element in the per-element-matrix K*u
     bool has_modulus = false, assembled = false;
     std::vector<DevLevel> lv;
     CoarseSolver coarse;
@@ -58,14 +68,21 @@ struct Solver {
     void set_modulus(const double *dE);
     void assemble();
     template <typename T>
-    void kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T *dg);
+    bool kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T *dg, int dot_slot = -1,
+                    T *xout = nullptr);
+    bool fine_tiled() const { return !super && kfine_kernel == 0 && walsh_ok; }
     template <typename T>
     void restrict_level(int c, const T *rf, T *rc);
     template <typename T>
     void prolong_level(int c, const T *xc, T *xf, bool add);
     template <typename T>
     void vcycle_t(const T *r0, T *z0);
-    void vcycle(const double *r0, double *z0);
+    bool vcycle(const double *r0, double *z0, int dot_slot = -1);
+    int vc_dot_slot = -1;  // finest post-smoothing also reduces z.r into pw.scal[vc_dot_slot]
+    bool vc_dotted = false;
+    DevBuf<double> dotp;   // per-workgroup partial sums of the fused dots
+    // compliance-gradient work space (tg_compliance_grad)
+    DevBuf<double> adjF, adjXp, adjE, adjCe, adjG, adjPart, adjRes;
     void pcg(const double *b_c, double *y_c, double tol, int maxIT, int warm, int *its, double *res);
 };
 

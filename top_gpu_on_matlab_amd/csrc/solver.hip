@@ -303,6 +303,22 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
 Solver::~Solver() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (auto &t : tpool) {
+        (void)hipEventDestroy(t.ev[0]);
+        (void)hipEventDestroy(t.ev[1]);
+    }
+}
+
+hipEvent_t *Solver::timing_slot(int key) {
+    if (tused == tpool.size()) {
+        TimedLaunch t;
+        TG_HIP(hipEventCreate(&t.ev[0]), "tg_solver:timing");
+        TG_HIP(hipEventCreate(&t.ev[1]), "tg_solver:timing");
+        tpool.push_back(t);
+    }
+    TimedLaunch &t = tpool[tused++];
+    t.key = key;
+    return t.ev;
 }
 
 void Solver::set_modulus(const double *dE) {
@@ -376,23 +392,51 @@ void Solver::assemble() {
     assembled = true;
 }
 
-template <typename T>
-static void launch_walsh(int epi, bool ck, unsigned gt, hipStream_t st, const T *u, T *out, const T *E,
-                         const uint8_t *mask, const KWVals<T> &kw, T cs, const Grid &g, const KTiles &kt,
-                         const T *r, const T *d, T w) {
-    if (ck) {
-        if (epi == EPI_KU) k_kfine_walsh<T, EPI_KU, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
-        else if (epi == EPI_RESID) k_kfine_walsh<T, EPI_RESID, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
-        else k_kfine_walsh<T, EPI_JACOBI, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
-    } else {
-        if (epi == EPI_KU) k_kfine_walsh<T, EPI_KU, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
-        else if (epi == EPI_RESID) k_kfine_walsh<T, EPI_RESID, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
-        else k_kfine_walsh<T, EPI_JACOBI, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w);
+template <typename T, bool MASKED>
+static void launch_kelem_vec(int parts, hipStream_t st, const T *u, T *fv, const T *Km, int64_t ldk,
+                             const uint8_t *eleMask, const uint8_t *nodeMask, const Grid &g) {
+    const unsigned ge = grid_for(parts * ldk, 256);
+    switch (parts) {
+        case 4: k_kelem_vec<T, MASKED, 4><<<ge, 256, 0, st>>>(u, fv, Km, ldk, eleMask, nodeMask, g); break;
+        case 6: k_kelem_vec<T, MASKED, 6><<<ge, 256, 0, st>>>(u, fv, Km, ldk, eleMask, nodeMask, g); break;
+        case 8: k_kelem_vec<T, MASKED, 8><<<ge, 256, 0, st>>>(u, fv, Km, ldk, eleMask, nodeMask, g); break;
+        case 12: k_kelem_vec<T, MASKED, 12><<<ge, 256, 0, st>>>(u, fv, Km, ldk, eleMask, nodeMask, g); break;
+        default: k_kelem_vec<T, MASKED, 24><<<ge, 256, 0, st>>>(u, fv, Km, ldk, eleMask, nodeMask, g); break;
     }
 }
 
+template <typename T, bool CK, bool DOT>
+static void launch_walsh_k(int epi, unsigned gt, hipStream_t st, const T *u, T *out, const T *E, const uint8_t *mask,
+                           const KWVals<T> &kw, T cs, const Grid &g, const KTiles &kt, const T *r, const T *d, T w,
+                           double *partial, T *xout) {
+    if (epi == EPI_KU)
+        k_kfine_walsh<T, EPI_KU, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+    else if (epi == EPI_RESID)
+        k_kfine_walsh<T, EPI_RESID, CK, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+    else if (epi == EPI_RESID_JAC0)
+        k_kfine_walsh<T, EPI_RESID_JAC0, CK, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+    else
+        k_kfine_walsh<T, EPI_JACOBI, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+}
+
 template <typename T>
-void Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T *dg) {
+static void launch_walsh(int epi, bool ck, unsigned gt, hipStream_t st, const T *u, T *out, const T *E,
+                         const uint8_t *mask, const KWVals<T> &kw, T cs, const Grid &g, const KTiles &kt,
+                         const T *r, const T *d, T w, double *partial, T *xout) {
+    if (ck) {
+        if (partial) launch_walsh_k<T, true, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+        else launch_walsh_k<T, true, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+    } else {
+        if (partial) launch_walsh_k<T, false, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+        else launch_walsh_k<T, false, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+    }
+}
+
+// dot_slot >= 0 (finest level, tiled kernel, EPI_KU or EPI_JACOBI only): also leave u.out (K*u) or
+// r.out (Jacobi) in pw.scal[dot_slot] and return true; otherwise return false.
+template <typename T>
+bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T *dg, int dot_slot, T *xout) {
+    bool dotted = false;
     DevLevel &D = lv[i];
     const unsigned gr = grid_for(D.g.nodes(), 256);
     const T w = (T)omega;
@@ -404,19 +448,33 @@ void Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
         if (kfine_kernel == 0 && walsh_ok) {
             const KTiles kt = ktiles_for(D.g, kfine_zchunk);
             const unsigned gt = (unsigned)kt.count();
+            hipEvent_t *ev = time_fine ? timing_slot((f64 ? 0 : 4) + epi) : nullptr;
+            double *dp = nullptr;
+            if (dot_slot >= 0 && epi != EPI_RESID) {
+                if ((int64_t)dotp.n < (int64_t)gt) dotp.alloc(gt);
+                dp = dotp.p;
+            }
+            if (ev) TG_HIP(hipEventRecord(ev[0], st), "tg_solver:kbyu");
             if (f64) {
                 const KWVals<double> &k = kw64;
                 const double *uu = (const double *)u, *r2 = (const double *)rr, *d2 = (const double *)dg;
                 const double *E2 = (const double *)E;
                 double *o2 = (double *)out;
-                launch_walsh<double>(epi, const_k0, gt, st, uu, o2, E2, D.nodeMask, k, model.prm.cell_size, D.g, kt, r2, d2, omega);
+                launch_walsh<double>(epi, const_k0, gt, st, uu, o2, E2, D.nodeMask, k, model.prm.cell_size, D.g, kt, r2, d2, omega, dp,
+                                     (double *)xout);
             } else {
                 const KWVals<float> &k = kw32;
                 const float *uu = (const float *)u, *r2 = (const float *)rr, *d2 = (const float *)dg;
                 const float *E2 = (const float *)E;
                 float *o2 = (float *)out;
                 const float w32 = (float)omega;
-                launch_walsh<float>(epi, const_k0, gt, st, uu, o2, E2, D.nodeMask, k, (float)model.prm.cell_size, D.g, kt, r2, d2, w32);
+                launch_walsh<float>(epi, const_k0, gt, st, uu, o2, E2, D.nodeMask, k, (float)model.prm.cell_size, D.g, kt, r2, d2, w32, dp,
+                                    (float *)xout);
+            }
+            if (ev) TG_HIP(hipEventRecord(ev[1], st), "tg_solver:kbyu");
+            if (dp) {
+                k_reduce_final<<<1, RED_THREADS, 0, st>>>(dp, (int)gt, pw.scal, dot_slot);
+                dotted = true;
             }
         } else if (const_k0) {
             if (epi == EPI_KU) k_kfine<T, EPI_KU, true><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, cs, D.g, rr, dg, w);
@@ -430,29 +488,33 @@ void Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
     } else {
         const T *Km = f64 ? (const T *)D.Ks64.p : (const T *)D.Ks32.p;
         T *fv = f64 ? (T *)D.fv64.p : (T *)D.fv32.p;
-        const unsigned ge = grid_for(KE_PARTS * D.ldk, 256);
         if (i == 0) {
-            k_kelem_vec<T, true><<<ge, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            launch_kelem_vec<T, true>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else k_kelem_gather<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
         } else {
-            k_kelem_vec<T, false><<<ge, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            launch_kelem_vec<T, false>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else k_kelem_gather<T, EPI_JACOBI, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
         }
     }
     TG_LAUNCH_CHECK("tg_solver:kbyu");
+    return dotted;
 }
 
 template <typename T>
 void Solver::restrict_level(int c, const T *rf, T *rc) {
     DevLevel &C = lv[c], &F = lv[c - 1];
     const T *cf = sizeof(T) == 8 ? (const T *)C.coeffF64.p : (const T *)C.coeffF32.p;
-    const unsigned gr = grid_for(8 * C.g.nodes(), 256);
-    if (C.span == 4) k_restrict<T, 4><<<gr, 256, 0, st>>>(rf, rc, cf, C.nodeMask, C.eleMask, C.g, F.g);
-    else k_restrict<T, 2><<<gr, 256, 0, st>>>(rf, rc, cf, C.nodeMask, C.eleMask, C.g, F.g);
+    T *fv = sizeof(T) == 8 ? (T *)C.fv64.p : (T *)C.fv32.p;
+    const unsigned gt = (unsigned)(((C.g.EY + RT_E - 1) / RT_E) * ((C.g.EX + RT_E - 1) / RT_E) *
+                                   ((C.g.EZ + RT_Z - 1) / RT_Z));
+    if (C.span == 4) k_restrict_tiled<T, 4><<<gt, RT_THREADS, 0, st>>>(rf, cf, C.g, F.g, fv, C.ldk);
+    else k_restrict_tiled<T, 2><<<gt, RT_THREADS, 0, st>>>(rf, cf, C.g, F.g, fv, C.ldk);
+    k_kelem_gather<T, EPI_KU, false><<<grid_for(C.g.nodes(), 256), 256, 0, st>>>(
+        fv, C.ldk, rc, nullptr, C.eleMask, C.nodeMask, C.g, nullptr, nullptr, (T)0);
     TG_LAUNCH_CHECK("tg_solver:restrict");
 }
 
@@ -493,6 +555,13 @@ void Solver::vcycle_t(const T *r0, T *z0) {
     for (int i = 0; i < L - 1; ++i) {
         DevLevel &D = lv[i];
         const int64_t nv = 3 * D.g.nodes();
+        if (i == 0 && !adapted && fine_tiled()) {
+            // pre-smoothing x = omega r ./ diag fused into the residual product
+            T *d = vec<T>(D, 2);
+            kbyu_level<T>(0, nullptr, d, EPI_RESID_JAC0, rr[0], diag_of<T>(D), -1, xx[0]);
+            restrict_level<T>(1, d, (T *)rr[1]);
+            continue;
+        }
         k_jacobi0<T><<<grid_for(nv, 256), 256, 0, st>>>(rr[i], diag_of<T>(D), xx[i], w, nv);
         if (adapted) {
             restrict_level<T>(i + 1, rr[i], (T *)rr[i + 1]);
@@ -513,23 +582,36 @@ void Solver::vcycle_t(const T *r0, T *z0) {
             const int64_t nv = 3 * D.g.nodes();
             k_jacobi_adapted<T><<<grid_for(nv, 256), 256, 0, st>>>(xx[i], rr[i], diag_of<T>(D), out, w, nv);
         } else {
-            kbyu_level<T>(i, xx[i], out, EPI_JACOBI, rr[i], diag_of<T>(D));
+            vc_dotted = kbyu_level<T>(i, xx[i], out, EPI_JACOBI, rr[i], diag_of<T>(D), i == 0 ? vc_dot_slot : -1);
         }
         xx[i] = out;
     }
     TG_LAUNCH_CHECK("tg_solver:vcycle");
 }
 
-void Solver::vcycle(const double *r0, double *z0) {
+// z0 = M r0.  dot_slot >= 0: also leave z0.r0 in pw.scal[dot_slot] when a fused kernel can (returns
+// true), so PCG skips its separate dot.
+bool Solver::vcycle(const double *r0, double *z0, int dot_slot) {
+    vc_dot_slot = -1;
+    vc_dotted = false;
     if (prec == TG_PREC_MIXED) {
         DevLevel &F = lv[0];
         const int64_t nv = 3 * F.g.nodes();
         k_convert<double, float><<<grid_for(nv, 256), 256, 0, st>>>(r0, rin32, nv);
         vcycle_t<float>(rin32, zout32);
+        if (dot_slot >= 0) {
+            k_convert_dot<<<RED_BLOCKS, RED_THREADS, 0, st>>>(zout32, r0, z0, nv, pw.partial);
+            k_reduce_final<<<1, RED_THREADS, 0, st>>>(pw.partial, RED_BLOCKS, pw.scal, dot_slot);
+            TG_LAUNCH_CHECK("tg_solver:vcycle");
+            return true;
+        }
         k_convert<float, double><<<grid_for(nv, 256), 256, 0, st>>>(zout32, z0, nv);
-    } else {
-        vcycle_t<double>(r0, z0);
+        return false;
     }
+    vc_dot_slot = dot_slot;
+    vcycle_t<double>(r0, z0);
+    vc_dot_slot = -1;
+    return vc_dotted;
 }
 
 // Solving_PCG (TOP_GPU.m:602-667) with Solving_KbyU_MatrixFree / Solving_Vcycle as operators.
@@ -545,9 +627,9 @@ void Solver::pcg(const double *b_c, double *y_c, double tol, int maxIT, int warm
     TG_LAUNCH_CHECK(id);
     int its = 0;
     double res = 0;
-    pw.run([&](const double *u, double *out) { kbyu_level<double>(0, u, out, EPI_KU, nullptr, nullptr); },
-           [&](const double *r, double *z) { vcycle(r, z); }, b, y, tol, maxIT, warm != 0, &its, &res,
-           "tg_solver_pcg:Numeric");
+    pw.run([&](const double *u, double *out, int slot) { return kbyu_level<double>(0, u, out, EPI_KU, nullptr, nullptr, slot); },
+           [&](const double *r, double *z, int slot) { return vcycle(r, z, slot); }, b, y, tol, maxIT, warm != 0, &its,
+           &res, "tg_solver_pcg:Numeric");
     k_pack<double, double><<<grid_for(F.Nn, 256), 256, 0, st>>>(y, y_c, F.nodFull0, F.Nn);
     TG_LAUNCH_CHECK(id);
     TG_HIP(hipEventRecord(ev1, st), id);
@@ -771,7 +853,34 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
         Solver &S = S_of(s, id);
         if (key == TG_TUNE_KFINE_KERNEL && (value == 0 || value == 1)) S.kfine_kernel = value;
         else if (key == TG_TUNE_KFINE_ZCHUNK && value >= 1 && value <= 4096) S.kfine_zchunk = value;
-        else fail(TG_ERR_ARGS, id, "unknown tuning key or value out of range");
+        else if (key == TG_TUNE_KELEM_PARTS && (value == 4 || value == 6 || value == 8 || value == 12 || value == 24))
+            S.kelem_parts = value;
+        else if (key == TG_TUNE_TIME_KFINE && (value == 0 || value == 1)) {
+            S.time_fine = value != 0;
+            S.tused = 0;
+        } else fail(TG_ERR_ARGS, id, "unknown tuning key or value out of range");
+    });
+}
+
+int tg_solver_kfine_timing(tg_solver *s, int32_t precision, int32_t epilogue, int64_t *launches, double *ms_total) {
+    return guard("tg_solver_kfine_timing:Input", [&] {
+        const char *id = "tg_solver_kfine_timing:Input";
+        Solver &S = S_of(s, id);
+        if (!launches || !ms_total || (precision != 0 && precision != 1) || epilogue < 0 || epilogue > 3)
+            fail(TG_ERR_ARGS, id, "bad arguments");
+        TG_HIP(hipStreamSynchronize(S.st), id);
+        const int key = 4 * precision + epilogue;
+        int64_t cnt = 0;
+        double tot = 0;
+        for (size_t i = 0; i < S.tused; ++i) {
+            if (S.tpool[i].key != key) continue;
+            float ms = 0;
+            TG_HIP(hipEventElapsedTime(&ms, S.tpool[i].ev[0], S.tpool[i].ev[1]), id);
+            tot += ms;
+            ++cnt;
+        }
+        *launches = cnt;
+        *ms_total = tot;
     });
 }
 

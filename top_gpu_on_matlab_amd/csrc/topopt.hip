@@ -382,10 +382,14 @@ void Filter::apply(const double *src, double *tar, double tol, int maxIT, int *i
     k_e2n<<<gn, 256, 0, st>>>(src, eleFwd, nodeMask, g, b);
     y.zero(st);
     TG_LAUNCH_CHECK(id);
-    pw.run([&](const double *u, double *out) {
+    pw.run([&](const double *u, double *out, int) {
                k_filter_apply<0><<<gn, 256, 0, st>>>(u, out, Kd, eleMask, nodeMask, g, nullptr);
+               return false;
            },
-           [&](const double *r, double *z) { k_filter_apply<1><<<gn, 256, 0, st>>>(r, z, nullptr, nullptr, nullptr, g, inv); },
+           [&](const double *r, double *z, int) {
+               k_filter_apply<1><<<gn, 256, 0, st>>>(r, z, nullptr, nullptr, nullptr, g, inv);
+               return false;
+           },
            b, y, tol, maxIT, false, its, res, id);
     k_n2e<<<grid_for(Ne, 256), 256, 0, st>>>(y, eleFull0, Ne, g, tar);
     TG_LAUNCH_CHECK(id);
@@ -449,6 +453,73 @@ int tg_solver_unit_compliance(tg_solver *s, const double *d_U, double *d_ce) {
         }
         TG_LAUNCH_CHECK(id);
         TG_HIP(hipStreamSynchronize(S.st), id);
+    });
+}
+
+int tg_compliance_grad(tg_solver *s, tg_filter *f, const double *d_x, double Emin, double E0, double penal,
+                       double tol, int32_t max_it, double filter_tol, int32_t warm, double *d_U, double *c_out,
+                       double *d_dcdx, int32_t *its_out) {
+    return guard("tg_compliance_grad:Input", [&] {
+        const char *id = "tg_compliance_grad:Input";
+        if (!s || !s->s || !d_x || !d_U || !c_out || max_it < 1) fail(TG_ERR_ARGS, id, "bad arguments");
+        Solver &S = *s->s;
+        if (S.super) fail(TG_ERR_STATE, "tg_compliance_grad:State", "standard voxel mode only");
+        if (f && (!f->f || f->f->Ne != S.lv[0].Ne)) fail(TG_ERR_ARGS, id, "filter does not match the solver's mesh");
+        const hipStream_t st = S.st;
+        const int64_t Ne = S.lv[0].Ne, ndof = 3 * S.lv[0].Nn;
+        if ((int64_t)S.adjF.n != ndof) {
+            S.adjF.upload(S.model.F.data(), ndof, st);  // load case 1
+            S.adjXp.alloc(Ne);
+            S.adjE.alloc(Ne);
+            S.adjCe.alloc(Ne);
+            S.adjG.alloc(Ne);
+            S.adjPart.alloc(RED_BLOCKS);
+            S.adjRes.alloc(1);
+        }
+        int fits = 0;
+        double fres = 0;
+        // xPhys = filter(x)
+        if (f) {
+            TG_HIP(hipStreamSynchronize(st), id);
+            f->f->apply(d_x, S.adjXp, filter_tol, max_it, &fits, &fres);
+            TG_HIP(hipStreamSynchronize(f->f->st), id);
+        } else {
+            TG_HIP(hipMemcpyAsync(S.adjXp.p, d_x, Ne * sizeof(double), hipMemcpyDeviceToDevice, st), id);
+        }
+        // E, K(E) U = F
+        k_simp<<<grid_for(Ne, 256), 256, 0, st>>>(S.adjXp, S.adjE, Ne, Emin, E0, penal);
+        TG_LAUNCH_CHECK(id);
+        S.set_modulus(S.adjE);
+        S.assemble();
+        int its = 0;
+        double res = 0;
+        S.pcg(S.adjF, d_U, tol, max_it, warm, &its, &res);
+        // ce_e = u_e' K0 u_e, c = sum E_e ce_e, g = -p (E0 - Emin) xPhys^(p-1) ce
+        DevLevel &F = S.lv[0];
+        S.pw.z.zero(st);
+        k_unpack<double, double><<<grid_for(F.Nn, 256), 256, 0, st>>>(d_U, S.pw.z, F.nodFull0, F.Nn);
+        k_unit_compliance<0><<<grid_for(Ne, 256), 256, 0, st>>>(S.pw.z, S.K0_64, nullptr, 0, F.eleFull0, Ne, F.g, S.adjCe);
+        TG_LAUNCH_CHECK(id);
+        double *h = nullptr;
+        TG_HIP(hipHostMalloc((void **)&h, sizeof(double), hipHostMallocDefault), id);
+        try {
+            *c_out = reduce_dev(R_DOT, S.adjE, S.adjCe, Ne, S.adjPart, S.adjRes, h, st);
+        } catch (...) {
+            (void)hipHostFree(h);
+            throw;
+        }
+        (void)hipHostFree(h);
+        if (d_dcdx) {
+            k_sens<<<grid_for(Ne, 256), 256, 0, st>>>(S.adjXp, S.adjCe, f ? S.adjG.p : d_dcdx, Ne, 1, Emin, E0, penal, 1.0);
+            TG_LAUNCH_CHECK(id);
+            if (f) {  // the Helmholtz filter is symmetric: its adjoint is the forward filter
+                TG_HIP(hipStreamSynchronize(st), id);
+                f->f->apply(S.adjG, d_dcdx, filter_tol, max_it, &fits, &fres);
+                TG_HIP(hipStreamSynchronize(f->f->st), id);
+            }
+        }
+        TG_HIP(hipStreamSynchronize(st), id);
+        if (its_out) *its_out = its;
     });
 }
 

@@ -118,13 +118,30 @@ class Solver:
                                         ctypes.byref(ms)))
         return ms.value
 
-    def set_tuning(self, kfine_kernel=None, kfine_zchunk=None):
-        """Finest-level K*u launch choice: kernel 'tiled' (default) or 'gather'; z planes per tile."""
-        from ._lib import TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK
+    def set_tuning(self, kfine_kernel=None, kfine_zchunk=None, kelem_parts=None):
+        """Finest-level K*u launch choice: kernel 'tiled' (default) or 'gather'; z planes per tile;
+        threads per element of the per-element-matrix K*u."""
+        from ._lib import TG_TUNE_KELEM_PARTS, TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK
+        if kelem_parts is not None:
+            check(lib().tg_solver_set_tuning(self.h, TG_TUNE_KELEM_PARTS, int(kelem_parts)))
         if kfine_kernel is not None:
             check(lib().tg_solver_set_tuning(self.h, TG_TUNE_KFINE_KERNEL, {'tiled': 0, 'gather': 1}[kfine_kernel]))
         if kfine_zchunk is not None:
             check(lib().tg_solver_set_tuning(self.h, TG_TUNE_KFINE_ZCHUNK, int(kfine_zchunk)))
+
+    def time_kfine(self, on=True):
+        """Start (clearing the record) or stop bracketing each finest-level K*u launch with HIP events."""
+        from ._lib import TG_TUNE_TIME_KFINE
+        check(lib().tg_solver_set_tuning(self.h, TG_TUNE_TIME_KFINE, 1 if on else 0))
+
+    def kfine_timing(self, precision='fp64', epilogue='ku'):
+        """(launches, total ms) of the timed finest K*u launches; epilogue 'ku', 'resid', 'jacobi' or
+        'resid_jac0' (residual with the pre-smoothing fused in)."""
+        n = ctypes.c_int64()
+        ms = ctypes.c_double()
+        check(lib().tg_solver_kfine_timing(self.h, 0 if precision == 'fp64' else 1,
+                                           {'ku': 0, 'resid': 1, 'jacobi': 2, 'resid_jac0': 3}[epilogue], ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
 
     def last_pcg_ms(self):
         ms = ctypes.c_double()

@@ -98,3 +98,18 @@ def test_model_errors_and_pairing():
     m.set_conditions([[1, 1, 1, 1], [interior, 1, 1, 1]], [[m.num_nodes, 0, 0, -1]])
     m.finalize()
     assert m.get_table(1, 'fixedDOFs')[3 * (interior - 1)] == 1
+
+
+def test_solver_level_entry_points_reject_bad_arguments_without_a_device():
+    """Null handles and bad options are rejected before any device work (status + identifier)."""
+    import ctypes
+    L = lib_mod.lib()
+    c = ctypes.c_double()
+    its = ctypes.c_int32()
+    st = L.tg_compliance_grad(None, None, None, 1e-6, 1.0, 3.0, 1e-3, 800, 1e-4, 0, None, ctypes.byref(c), None,
+                              ctypes.byref(its))
+    assert st == 1 and L.tg_last_error_id() == b'tg_compliance_grad:Input'
+    assert L.tg_solver_set_tuning(None, 1, 0) == 1
+    n = ctypes.c_int64()
+    assert L.tg_solver_kfine_timing(None, 0, 0, ctypes.byref(n), ctypes.byref(c)) == 1
+    assert L.tg_last_error_id() == b'tg_solver_kfine_timing:Input'

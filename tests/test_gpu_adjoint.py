@@ -33,7 +33,7 @@ def _fd_check(shape, precision, n_vox, seed, rMin):
     c.backward()
     g = xr.grad.clone()
     U0 = op.U.clone()
-    c0 = float(c)
+    c0 = float(c.detach())
     # the reference's normalised sensitivity, filtered: dc_ref = filter(-p (E0-Emin) xPhys^(p-1) ce / cSolid)
     xPhys = filt(x)
     ce = G.unit_compliance(op.solver, U0)

@@ -50,11 +50,10 @@ __global__ void k_coarse_assemble(const double *__restrict__ Ks, int64_t ldk, co
     else As[asOff[pJ] + row + np * col] = val;
 }
 
-__global__ void k_lower_only(const double *__restrict__ S, double *__restrict__ L, int64_t n) {
+__global__ void k_identity(double *__restrict__ X, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * n) return;
-    const int64_t r = i % n, c = i / n;
-    L[i] = (r >= c) ? S[i] : 0.0;
+    X[i] = (i % n == i / n) ? 1.0 : 0.0;
 }
 
 // Row-major copy of a column-major n x n block: Tr[r * n + c] = A[r + n * c].
@@ -224,6 +223,10 @@ void rb(rocblas_status s, const char *what) {
 CoarseSolver::~CoarseSolver() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (h) rocblas_destroy_handle(h);
+    if (h2) rocblas_destroy_handle(h2);
+    if (evFork) (void)hipEventDestroy(evFork);
+    if (evJoin) (void)hipEventDestroy(evJoin);
+    if (st2) (void)hipStreamDestroy(st2);
 }
 
 void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
@@ -294,6 +297,10 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
     xx.alloc(ntot);
     info.alloc(2 * P);
     if (!h) rb(rocblas_create_handle(&h), "rocblas_create_handle");
+    if (!h2) rb(rocblas_create_handle(&h2), "rocblas_create_handle");
+    if (!st2) TG_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking), "tg_solver:coarse");
+    if (!evFork) TG_HIP(hipEventCreateWithFlags(&evFork, hipEventDisableTiming), "tg_solver:coarse");
+    if (!evJoin) TG_HIP(hipEventCreateWithFlags(&evJoin, hipEventDisableTiming), "tg_solver:coarse");
     TG_HIP(hipStreamSynchronize(st), "tg_solver:coarse");
 }
 
@@ -312,39 +319,46 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
     // As_p (n_p x n_{p-1}, column-major) holds A_{p,p-1}.  Chain A turns As_{p+1} into
     // L_{p+1,p} = A_{p+1,p} L_pp^-T (right trsm); chain B turns As_p into X_p = L_pp^-1 A_{p,p-1}
     // = L_{p-1,p}^T (left trsm).
-    auto factor_plane = [&](int p) {
+    // Plane factor (rocSOLVER potrf) and its inverse inv(L) = L \ I by one triangular solve.  Chains A
+    // and B are independent until the separator: A runs on the solver stream, B on a second stream.
+    auto factor_plane = [&](int p, rocblas_handle hh, hipStream_t ss) {
         const rocblas_int np = (rocblas_int)nd[p];
         double *S = Ad.p + adOff[p];
-        rb(rocsolver_dpotrf(h, rocblas_fill_lower, np, S, np, info.p + p), "rocsolver_dpotrf");
-        k_lower_only<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(S, Linv.p + adOff[p], np);
+        rb(rocsolver_dpotrf(hh, rocblas_fill_lower, np, S, np, info.p + p), "rocsolver_dpotrf");
+        k_identity<<<grid_for((int64_t)np * np, 256), 256, 0, ss>>>(Linv.p + adOff[p], np);
         TG_LAUNCH_CHECK(id);
-        rb(rocsolver_dtrtri(h, rocblas_fill_lower, rocblas_diagonal_non_unit, np, Linv.p + adOff[p], np, info.p + P + p),
-           "rocsolver_dtrtri");
+        rb(rocblas_dtrsm(hh, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+                         np, np, &one, S, np, Linv.p + adOff[p], np), "rocblas_dtrsm inverse");
     };
+    TG_HIP(hipEventRecord(evFork, st), id);
+    TG_HIP(hipStreamWaitEvent(st2, evFork, 0), id);
+    rb(rocblas_set_stream(h2, st2), "rocblas_set_stream");
     for (int p = 0; p < m; ++p) {  // chain A
         const rocblas_int np = (rocblas_int)nd[p];
         if (np == 0) continue;
         if (p > 0 && nd[p - 1] > 0)
             rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, np, (rocblas_int)nd[p - 1], &mone,
                              As.p + asOff[p], np, &one, Ad.p + adOff[p], np), "rocblas_dsyrk");
-        factor_plane(p);
+        factor_plane(p, h, st);
         if (nd[p + 1] > 0)
             rb(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
                              rocblas_diagonal_non_unit, (rocblas_int)nd[p + 1], np, &one, Ad.p + adOff[p], np,
                              As.p + asOff[p + 1], (rocblas_int)nd[p + 1]), "rocblas_dtrsm A");
     }
-    for (int p = P - 1; p > m; --p) {  // chain B
+    for (int p = P - 1; p > m; --p) {  // chain B, second stream
         const rocblas_int np = (rocblas_int)nd[p];
         if (np == 0) continue;
         if (p + 1 < P && nd[p + 1] > 0)  // S_p -= X_{p+1}^T X_{p+1}
-            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, np, (rocblas_int)nd[p + 1], &mone,
+            rb(rocblas_dsyrk(h2, rocblas_fill_lower, rocblas_operation_transpose, np, (rocblas_int)nd[p + 1], &mone,
                              As.p + asOff[p + 1], (rocblas_int)nd[p + 1], &one, Ad.p + adOff[p], np), "rocblas_dsyrk");
-        factor_plane(p);
+        factor_plane(p, h2, st2);
         if (nd[p - 1] > 0)
-            rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+            rb(rocblas_dtrsm(h2, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                              rocblas_diagonal_non_unit, np, (rocblas_int)nd[p - 1], &one, Ad.p + adOff[p], np,
                              As.p + asOff[p], np), "rocblas_dtrsm B");
     }
+    TG_HIP(hipEventRecord(evJoin, st2), id);
+    TG_HIP(hipStreamWaitEvent(st, evJoin, 0), id);
     if (nd[m] > 0) {  // separator
         const rocblas_int nm = (rocblas_int)nd[m];
         if (m > 0 && nd[m - 1] > 0)
@@ -353,7 +367,7 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
         if (m + 1 < P && nd[m + 1] > 0)
             rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, nm, (rocblas_int)nd[m + 1], &mone,
                              As.p + asOff[m + 1], (rocblas_int)nd[m + 1], &one, Ad.p + adOff[m], nm), "rocblas_dsyrk");
-        factor_plane(m);
+        factor_plane(m, h, st);
     }
     // Solve operators, row-major with zero-padded rows (column-major results of transposed products):
     //   LT_p = inv(L_pp) (row-major copy); inv(L_pp)^T is the column-major Linv itself.
@@ -362,12 +376,16 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
     //            m: [inv(L_mm) L_{m,m-1} | inv(L_mm) L_{m,m+1}]
     //   backward A: inv(L_pp)^T L_{p+1,p}^T   = (As_{p+1} Linv_p)^T      (n_p x n_{p+1})
     //            B: inv(L_pp)^T L_{p-1,p}^T   = (X_p^T Linv_p)^T         (n_p x n_{p-1})
+    TG_HIP(hipEventRecord(evFork, st), id);
+    TG_HIP(hipStreamWaitEvent(st2, evFork, 0), id);
     for (int p = 0; p < P; ++p) {
         const rocblas_int np = (rocblas_int)nd[p];
         if (np == 0) continue;
         const int64_t prev = p > 0 ? nd[p - 1] : 0, next = p + 1 < P ? nd[p + 1] : 0;
         const double *Lp = Linv.p + adOff[p];
-        k_transpose_block<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Lp, LT.p + adOff[p], np);
+        rocblas_handle h = p > m ? h2 : this->h;
+        hipStream_t ss = p > m ? st2 : st;
+        k_transpose_block<<<grid_for((int64_t)np * np, 256), 256, 0, ss>>>(Lp, LT.p + adOff[p], np);
         TG_LAUNCH_CHECK(id);
         double *F = Fop.p + fOff[p], *B = Bop.p + bOff[p];
         const rocblas_int fl = (rocblas_int)fLd[p], bl = (rocblas_int)bLd[p];
@@ -385,6 +403,8 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
             rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)prev, np, np, &one,
                              As.p + asOff[p], np, Lp, np, &zero, B, bl), "rocblas_dgemm B B");
     }
+    TG_HIP(hipEventRecord(evJoin, st2), id);
+    TG_HIP(hipStreamWaitEvent(st, evJoin, 0), id);
     std::vector<rocblas_int> hinfo(2 * P);
     TG_HIP(hipMemcpyAsync(hinfo.data(), info.p, sizeof(rocblas_int) * 2 * P, hipMemcpyDeviceToHost, st), id);
     TG_HIP(hipStreamSynchronize(st), id);

@@ -404,20 +404,34 @@ __global__ __launch_bounds__(256) void k_prolong(const T *__restrict__ xc, T *__
     const int64_t t = i / gf.NY;
     const int ix = (int)(t % gf.NX), iz = (int)(t / gf.NX);
     T a0 = 0, a1 = 0, a2 = 0;
+    T o0 = 0, o1 = 0, o2 = 0;
+    if (ADD) {  // issued first: independent of the element search
+        o0 = xf[3 * i];
+        o1 = xf[3 * i + 1];
+        o2 = xf[3 * i + 2];
+    }
     if (fMask[i] & NODE_ACTIVE) {
-        // coarse elements containing the node: e in [ceil(i/S) - 1, floor(i/S)], clamped to the grid
+        // coarse elements containing the node: e in [ceil(i/S) - 1, floor(i/S)], clamped to the grid;
+        // the (up to 8) solidity flags are loaded together, then the first solid one (z, x, y order)
         const int qy = min(iy / S, gc.EY - 1), qx = min(ix / S, gc.EX - 1), qz = min(iz / S, gc.EZ - 1);
         const int ny0 = max((iy + S - 1) / S - 1, 0), nx0 = max((ix + S - 1) / S - 1, 0);
         const int nz0 = max((iz + S - 1) / S - 1, 0);
+        uint8_t fl[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int cy = (k & 1) ? qy : ny0, cx = (k & 2) ? qx : nx0, cz = (k & 4) ? qz : nz0;
+            fl[k] = cEle[cy + (int64_t)gc.EY * (cx + (int64_t)gc.EX * cz)];
+        }
         int ey = -1, ex = -1, ez = -1;
-        for (int cz = nz0; cz <= qz && ez < 0; ++cz)
-            for (int cx = nx0; cx <= qx && ez < 0; ++cx)
-                for (int cy = ny0; cy <= qy && ez < 0; ++cy)
-                    if (cEle[cy + (int64_t)gc.EY * (cx + (int64_t)gc.EX * cz)]) {
-                        ey = cy;
-                        ex = cx;
-                        ez = cz;
-                    }
+#pragma unroll
+        for (int k = 7; k >= 0; --k) {  // descending, so the last assignment is the first solid candidate
+            const int cy = (k & 1) ? qy : ny0, cx = (k & 2) ? qx : nx0, cz = (k & 4) ? qz : nz0;
+            if (fl[k]) {
+                ey = cy;
+                ex = cx;
+                ez = cz;
+            }
+        }
         if (ez >= 0) {
             const int a = iy - S * ey, b = ix - S * ex, c = iz - S * ez;
 #pragma unroll
@@ -432,9 +446,9 @@ __global__ __launch_bounds__(256) void k_prolong(const T *__restrict__ xc, T *__
         }
     }
     if (ADD) {
-        xf[3 * i] += a0;
-        xf[3 * i + 1] += a1;
-        xf[3 * i + 2] += a2;
+        xf[3 * i] = o0 + a0;
+        xf[3 * i + 1] = o1 + a1;
+        xf[3 * i + 2] = o2 + a2;
     } else {
         xf[3 * i] = a0;
         xf[3 * i + 1] = a1;

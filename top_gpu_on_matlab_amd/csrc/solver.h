@@ -48,14 +48,17 @@ struct Solver {
     std::vector<TimedLaunch> tpool;
     size_t tused = 0;
     hipEvent_t *timing_slot(int key);
-    int kfine_zchunk = 16;  // node planes per workgroup of the tiled kernel
-    int kelem_parts = 4;    // threads per element in the per-element-matrix K*u
+    int kfine_zchunk = 12;  // node planes per workgroup of the tiled kernel
+    int kelem_parts = 24;   // threads per element in the per-element-matrix K*u
     bool has_modulus = false, assembled = false;
     std::vector<DevLevel> lv;
     CoarseSolver coarse;
     DevBuf<double> K0_64, E64, Esub, Imat, uFree, uFixed;
     DevBuf<float> K0_32, E32;
     DevBuf<int32_t> fullUid;
+    DevBuf<double> Gt, Ech;                // level-2 Galerkin GEMM: G_s' (nCh x 576), children moduli
+    DevBuf<int64_t> bndList;               // coarse elements with a boundary (fixed-DOF) child
+    int64_t nBnd = 0;
     DevBuf<double> b, y;        // PCG right-hand side and iterate (full fine grid)
     PcgWork pw;                 // PCG work vectors
     DevBuf<float> rin32, zout32;

@@ -76,6 +76,34 @@ __global__ __launch_bounds__(GK_THREADS) void k_galerkin_dev(Src src, const uint
     for (int i = 0; i < 9; ++i) Ks[(int64_t)galerkin_entry(i) * ldk + ce] = acc[i];
 }
 
+// The same product restricted to a list of coarse elements (those with boundary children).
+template <class Src>
+__global__ __launch_bounds__(GK_THREADS) void k_galerkin_list(Src src, const int64_t *__restrict__ list, int nCh,
+                                                              const double *__restrict__ P, double *__restrict__ Ks,
+                                                              int64_t ldk) {
+    __shared__ double sB[576], sK[576], sT[576];
+    const int64_t ce = list[blockIdx.x];
+    src.ce = ce;
+    double acc[9];
+    galerkin_block(acc, src, nCh, P, sB, sK, sT);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ks[(int64_t)galerkin_entry(i) * ldk + ce] = acc[i];
+}
+
+// Children moduli of every coarse element, child-major: Ech[ce + ldk * s] = E(child s) (0 when void).
+__global__ void k_gather_children(const double *__restrict__ E, const uint8_t *__restrict__ fineSolid, int span,
+                                  int cEY, int cEX, int64_t nC, int fEY, int fEX, int nCh, double *__restrict__ Ech,
+                                  int64_t ldk) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nC * nCh) return;
+    const int64_t ce = t % nC;
+    const int s = (int)(t / nC);
+    const int a = s % span, b = (s / span) % span, c = s / (span * span);
+    const int cy = (int)(ce % cEY), cx = (int)((ce / cEY) % cEX), cz = (int)(ce / ((int64_t)cEY * cEX));
+    const int64_t f = (int64_t)(span * cy + a) + (int64_t)fEY * ((span * cx + b) + (int64_t)fEX * (span * cz + c));
+    Ech[ce + ldk * s] = fineSolid[f] ? E[f] : 0.0;
+}
+
 // K_mat = I_mat * E_sub per element (TOP_GPU.m:1101-1103), entry-major output.
 __global__ void k_kmat(const double *__restrict__ Imat, const double *__restrict__ Esub, double *__restrict__ Km,
                        int64_t ldk, int64_t nE) {
@@ -268,6 +296,47 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
     std::vector<int32_t> uid(nEf, 0);
     for (int64_t k = 0; k < F.Ne; ++k) uid[m.lv[0].eleMapBack[k] - 1] = m.mapUniqueKes[k];
     fullUid.upload(uid, st);
+    if (!super && L >= 2) {
+        // Level-2 Galerkin as a GEMM: Ks_e = sum_s E_s G_s with G_s = Pch_s' K0 Pch_s for children
+        // without fixed DOFs; coarse elements with a boundary child are recomputed child by child.
+        DevLevel &C = lv[1];
+        const int span = C.span, nCh = span * span * span;
+        std::vector<double> Ph;
+        build_pch(span, Ph);
+        std::vector<double> gt((size_t)nCh * 576, 0.0), tmp(576);
+        for (int sc = 0; sc < nCh; ++sc) {
+            const double *Pc = Ph.data() + (size_t)576 * sc;
+            for (int c = 0; c < 24; ++c)          // tmp = K0 Pch_s
+                for (int a = 0; a < 24; ++a) {
+                    double v = 0;
+                    for (int b = 0; b < 24; ++b) v += m.Ke[a + 24 * b] * Pc[b + 24 * c];
+                    tmp[a + 24 * c] = v;
+                }
+            for (int c = 0; c < 24; ++c)          // G_s = Pch_s' tmp
+                for (int r = 0; r < 24; ++r) {
+                    double v = 0;
+                    for (int a = 0; a < 24; ++a) v += Pc[a + 24 * r] * tmp[a + 24 * c];
+                    gt[sc + (size_t)nCh * (r + 24 * c)] = v;
+                }
+        }
+        Gt.upload(gt, st);
+        std::vector<int64_t> bnd;
+        const Grid &cgd = C.g, &fgd = F.g;
+        for (int64_t ce = 0; ce < cgd.elems(); ++ce) {
+            const int cy = (int)(ce % cgd.EY), cx = (int)((ce / cgd.EY) % cgd.EX), cz = (int)(ce / ((int64_t)cgd.EY * cgd.EX));
+            bool b = false;
+            for (int sc = 0; sc < nCh && !b; ++sc) {
+                const int a = sc % span, bb2 = (sc / span) % span, c = sc / (span * span);
+                const int64_t f = (int64_t)(span * cy + a) + (int64_t)fgd.EY * ((span * cx + bb2) + (int64_t)fgd.EX * (span * cz + c));
+                b = uid[f] > 0;
+            }
+            if (b) bnd.push_back(ce);
+        }
+        nBnd = (int64_t)bnd.size();
+        if (nBnd) bndList.upload(bnd, st);
+        Ech.alloc((size_t)nCh * C.ldk);
+        Ech.zero(st);
+    }
     uFree.upload(m.uniqueFree, st);
     uFixed.upload(m.uniqueFixed, st);
     if (super) {
@@ -353,7 +422,16 @@ void Solver::assemble() {
             SrcDevL2 src;
             static_cast<ChildGrid &>(src) = cg;
             src.E = E64; src.K0 = K0_64; src.uf = uFree; src.ux = uFixed; src.uid = fullUid;
-            k_galerkin_dev<<<grid, GK_THREADS, 0, st>>>(src, C.eleMask, nCh, C.P, C.Ks64, C.ldk);
+            const int64_t nC = C.g.elems();
+            k_gather_children<<<grid_for(nC * nCh, 256), 256, 0, st>>>(E64, Fi.eleMask, C.span, C.g.EY, C.g.EX, nC,
+                                                                      Fi.g.EY, Fi.g.EX, nCh, Ech, C.ldk);
+            TG_LAUNCH_CHECK(id);
+            const double one = 1.0, zero = 0.0;
+            if (rocblas_set_stream(coarse.h, st) != rocblas_status_success ||
+                rocblas_dgemm(coarse.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nC, 576, nCh, &one,
+                              Ech, (rocblas_int)C.ldk, Gt, nCh, &zero, C.Ks64, (rocblas_int)C.ldk) != rocblas_status_success)
+                fail(TG_ERR_HIP, "tg_solver:rocblas", "level-2 Galerkin GEMM failed");
+            if (nBnd) k_galerkin_list<<<(unsigned)nBnd, GK_THREADS, 0, st>>>(src, bndList, nCh, C.P, C.Ks64, C.ldk);
         } else if (i == 1) {
             SrcDevSuper src;
             static_cast<ChildGrid &>(src) = cg;

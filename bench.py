@@ -28,10 +28,10 @@ import numpy as np
 METRIC = 'PCG solves/sec + K·u HBM GB/s, mixed vs FP64 at 1e-6 rel compliance err'
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per launch of the FP64 finest K*u at 128x256x128, from two rocprofv3 --pmc runs
-# (FETCH_SIZE, WRITE_SIZE) scaled by the same counters on a plain device copy of known size; the
-# runs and the factors are recorded in profiles/kfine_walsh_fp64_c5_128x256x128_pmc_traffic.md
-KFINE64_TRAFFIC = None
+# HBM bytes per launch of the FP64 finest K*u at 128x256x128 (bytes), from two rocprofv3 --pmc runs:
+# FETCH_SIZE 157,400 KiB x 2.0 + WRITE_SIZE 117,390 KiB x 1.0, the factors being what the counters
+# read for a plain 2 GiB device copy (FETCH 1,048,576 KiB, WRITE 2,097,152 KiB)
+KFINE64_TRAFFIC = 442.6e6
 
 
 def log(*a):

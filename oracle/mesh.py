@@ -383,14 +383,15 @@ def adapt_bc_external(srcBC, nely, nelx, nelz, resX, resY, resZ):
     return out
 
 
-def build_file_model(nelx, nely, nelz, voxels, fixing, loads, nu=0.3, coarsest_resolution_control=20000):
+def build_file_model(nelx, nely, nelz, voxels, fixing, loads, nu=0.3, coarsest_resolution_control=20000,
+                     non_dyadic=1):
     """CreateVoxelFEAmodel for a parsed .TopVoxel file (TOP_GPU.m:416-496): cellSize_ = 1 (TOP_GPU.m:67);
     fixations and loads are uniqued, sorted, moved to the padded grid and renumbered to solid nodes."""
     flat = np.zeros(nelx * nely * nelz, dtype=bool)
     flat[np.asarray(voxels, dtype=np.int64) - 1] = True
     vox = flat.reshape((nely, nelx, nelz), order='F')
     finest, numLevels, half, _ = voxel_discretization(vox, coarsest_resolution_control)
-    levels, numLevels, nd = build_mesh_hierarchy(finest, numLevels, half, 1)
+    levels, numLevels, nd = build_mesh_hierarchy(finest, numLevels, half, non_dyadic)
 
     def prep(a):
         a = np.asarray(a, dtype=np.float64)

@@ -1,4 +1,5 @@
 """GPU parity of the device solver (libtopgpu, through the C ABI) against the oracle."""
+import os
 import sys
 
 import numpy as np
@@ -14,6 +15,9 @@ CASES = {
     'box': dict(shape=(8, 16, 8), case=1, vox=None, se=False),
     'holes': dict(shape=(12, 20, 12), case=1, vox=holey_mask(12, 20, 12, seed=3), se=False),
     'mbb_super': dict(shape=(10, 20, 10), case=2, vox=None, se=True),
+    # the value-noise design domain (file input, boundary-node clamps, top-patch load) on a 48 x 24 x 24
+    # box, non-dyadic hierarchy
+    'value_noise': dict(shape=(24, 48, 24), case=None, vox=None, se=False, noise=dict(period=32, non_dyadic=1)),
 }
 
 
@@ -24,18 +28,29 @@ def _rel(a, b):
 
 
 @pytest.fixture(scope='module', params=list(CASES))
-def pair(request):
+def pair(request, tmp_path_factory):
     import torch
     from top_gpu_on_matlab_amd.model import Model
     from top_gpu_on_matlab_amd.solver import Solver
     c = CASES[request.param]
     ny, nx, nz = c['shape']
     print('\n[case %s shape %s]' % (request.param, c['shape']), flush=True)
-    vox = c['vox'] if c['vox'] is not None else np.ones(c['shape'], bool)
-    om = M.build_cuboid_model(ny, nx, nz, opt_case=c['case'], voxels=vox, super_element=c['se'])
-    gm = Model(vox, super_element=c['se'])
-    gm.builtin_case(c['case'])
-    gm.finalize()
+    if c.get('noise'):
+        from top_gpu_on_matlab_amd.synthetic import bottom_clamp_top_patch, value_noise_mask
+        from top_gpu_on_matlab_amd.topvoxel import load_topvoxel_model, read_topvoxel, write_topvoxel
+        mask = value_noise_mask(nx, ny, nz, seed=0, octaves=4, period=c['noise']['period'])
+        path = os.path.join(str(tmp_path_factory.mktemp('noise')), 'value_noise.TopVoxel')
+        write_topvoxel(path, mask, *bottom_clamp_top_patch(mask))
+        d = read_topvoxel(path)
+        om = M.build_file_model(d['nelx'], d['nely'], d['nelz'], d['voxels'], d['fixing'], d['loads'][0],
+                                non_dyadic=c['noise']['non_dyadic'])
+        gm = load_topvoxel_model(path, non_dyadic=c['noise']['non_dyadic'])
+    else:
+        vox = c['vox'] if c['vox'] is not None else np.ones(c['shape'], bool)
+        om = M.build_cuboid_model(ny, nx, nz, opt_case=c['case'], voxels=vox, super_element=c['se'])
+        gm = Model(vox, super_element=c['se'])
+        gm.builtin_case(c['case'])
+        gm.finalize()
     Ne = om.levels[0].numElements
     E = random_modulus(Ne * (64 if c['se'] else 1), seed=2)
     if c['se']:
@@ -132,8 +147,10 @@ def test_pcg_parity_and_determinism(pair):
     rounding (summation order in element products, transfers and the coarse solve), which CG
     amplifies; the bound is calibrated on the oracle itself: a relative perturbation of b at the
     unit-roundoff level (2^-52) moves the oracle's compliance b'y and iterate by some spread, and
-    the device must agree with the oracle to within 20x that spread (never looser than 1e-8 for the
-    compliance) with the same iteration count."""
+    the device must agree with the oracle to within 20x that spread with the same iteration count --
+    or, where the perturbed oracle's own count moves (a residual that stagnates near tol, as on the
+    value-noise domain, where the spread reaches ~1e-7 in compliance), within 3x that move.  Where
+    the oracle is well conditioned (spread below 5e-10) the compliance must agree to 1e-8."""
     name, om, osv, gm, gsv, E = pair
     Es = smooth_modulus(om.levels[0], per=64 if om.super_element else 1)
     osv = S.Solver(om)
@@ -152,14 +169,16 @@ def test_pcg_parity_and_determinism(pair):
     c_dev = abs(float(b @ y.cpu().numpy()) - c_ref) / abs(c_ref)
     print('\n[%s] its %d; oracle ulp-spread: compliance %.1e iterate %.1e; device: compliance %.1e iterate %.1e'
           % (name, its_ref, c_spread, y_spread, c_dev, _rel(y.cpu().numpy(), y_ref)), flush=True)
-    assert its == its_ref, (name, its, its_ref, hist[-3:], res)
-    assert c_dev < max(20 * c_spread, 1e-12) and c_dev < 1e-8
+    its_slack = 3 * abs(its_per - its_ref)
+    assert abs(its - its_ref) <= its_slack, (name, its, its_ref, its_per, hist[-3:], res)
+    assert c_dev < max(20 * c_spread, 1e-12)
+    assert c_dev < 1e-8 or c_spread > 5e-10
     assert _rel(y.cpu().numpy(), y_ref) < max(20 * y_spread, 1e-10)
     assert its2 == its and bool((y == y2).all()), 'PCG is not bitwise reproducible'
     # warm start from the previous solution (TOP_GPU.m:186)
     y3, its3, _ = gsv.pcg(b, y0=y, tol=1e-3, max_it=800)
     y3r, its3r, _ = osv.pcg(b, tol=1e-3, maxIT=800, y0=y.cpu().numpy())
-    assert its3 == its3r
+    assert abs(its3 - its3r) <= its_slack
     assert _rel(y3.cpu().numpy(), y3r) < max(20 * y_spread, 1e-10)
     gsv.set_modulus(np.asarray(E).reshape(-1, order='F'))      # restore the fixture's state
     gsv.assemble()

@@ -223,10 +223,7 @@ void rb(rocblas_status s, const char *what) {
 CoarseSolver::~CoarseSolver() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (h) rocblas_destroy_handle(h);
-    if (h2) rocblas_destroy_handle(h2);
-    if (evFork) (void)hipEventDestroy(evFork);
-    if (evJoin) (void)hipEventDestroy(evJoin);
-    if (st2) (void)hipStreamDestroy(st2);
+
 }
 
 void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
@@ -295,12 +292,8 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
     bb.alloc(ntot);
     yy.alloc(ntot);
     xx.alloc(ntot);
-    info.alloc(2 * P);
+    info.alloc(P);
     if (!h) rb(rocblas_create_handle(&h), "rocblas_create_handle");
-    if (!h2) rb(rocblas_create_handle(&h2), "rocblas_create_handle");
-    if (!st2) TG_HIP(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking), "tg_solver:coarse");
-    if (!evFork) TG_HIP(hipEventCreateWithFlags(&evFork, hipEventDisableTiming), "tg_solver:coarse");
-    if (!evJoin) TG_HIP(hipEventCreateWithFlags(&evJoin, hipEventDisableTiming), "tg_solver:coarse");
     TG_HIP(hipStreamSynchronize(st), "tg_solver:coarse");
 }
 
@@ -319,46 +312,44 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
     // As_p (n_p x n_{p-1}, column-major) holds A_{p,p-1}.  Chain A turns As_{p+1} into
     // L_{p+1,p} = A_{p+1,p} L_pp^-T (right trsm); chain B turns As_p into X_p = L_pp^-1 A_{p,p-1}
     // = L_{p-1,p}^T (left trsm).
-    // Plane factor (rocSOLVER potrf) and its inverse inv(L) = L \ I by one triangular solve.  Chains A
-    // and B are independent until the separator: A runs on the solver stream, B on a second stream.
-    auto factor_plane = [&](int p, rocblas_handle hh, hipStream_t ss) {
+    // Plane factor (rocSOLVER potrf) and its inverse inv(L) = L \ I by one triangular solve.  Every
+    // call runs on the solver stream: with the two chains on two streams (two rocBLAS handles) the
+    // factors came out different from run to run on ROCm 7.2 (wrong in the low to leading digits on
+    // about one assembly in eight), however the streams were ordered; on one stream they are
+    // bitwise reproducible.
+    auto factor_plane = [&](int p) {
         const rocblas_int np = (rocblas_int)nd[p];
         double *S = Ad.p + adOff[p];
-        rb(rocsolver_dpotrf(hh, rocblas_fill_lower, np, S, np, info.p + p), "rocsolver_dpotrf");
-        k_identity<<<grid_for((int64_t)np * np, 256), 256, 0, ss>>>(Linv.p + adOff[p], np);
+        rb(rocsolver_dpotrf(h, rocblas_fill_lower, np, S, np, info.p + p), "rocsolver_dpotrf");
+        k_identity<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Linv.p + adOff[p], np);
         TG_LAUNCH_CHECK(id);
-        rb(rocblas_dtrsm(hh, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
+        rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
                          np, np, &one, S, np, Linv.p + adOff[p], np), "rocblas_dtrsm inverse");
     };
-    TG_HIP(hipEventRecord(evFork, st), id);
-    TG_HIP(hipStreamWaitEvent(st2, evFork, 0), id);
-    rb(rocblas_set_stream(h2, st2), "rocblas_set_stream");
     for (int p = 0; p < m; ++p) {  // chain A
         const rocblas_int np = (rocblas_int)nd[p];
         if (np == 0) continue;
         if (p > 0 && nd[p - 1] > 0)
             rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, np, (rocblas_int)nd[p - 1], &mone,
                              As.p + asOff[p], np, &one, Ad.p + adOff[p], np), "rocblas_dsyrk");
-        factor_plane(p, h, st);
+        factor_plane(p);
         if (nd[p + 1] > 0)
             rb(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
                              rocblas_diagonal_non_unit, (rocblas_int)nd[p + 1], np, &one, Ad.p + adOff[p], np,
                              As.p + asOff[p + 1], (rocblas_int)nd[p + 1]), "rocblas_dtrsm A");
     }
-    for (int p = P - 1; p > m; --p) {  // chain B, second stream
+    for (int p = P - 1; p > m; --p) {  // chain B
         const rocblas_int np = (rocblas_int)nd[p];
         if (np == 0) continue;
         if (p + 1 < P && nd[p + 1] > 0)  // S_p -= X_{p+1}^T X_{p+1}
-            rb(rocblas_dsyrk(h2, rocblas_fill_lower, rocblas_operation_transpose, np, (rocblas_int)nd[p + 1], &mone,
+            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, np, (rocblas_int)nd[p + 1], &mone,
                              As.p + asOff[p + 1], (rocblas_int)nd[p + 1], &one, Ad.p + adOff[p], np), "rocblas_dsyrk");
-        factor_plane(p, h2, st2);
+        factor_plane(p);
         if (nd[p - 1] > 0)
-            rb(rocblas_dtrsm(h2, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+            rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
                              rocblas_diagonal_non_unit, np, (rocblas_int)nd[p - 1], &one, Ad.p + adOff[p], np,
                              As.p + asOff[p], np), "rocblas_dtrsm B");
     }
-    TG_HIP(hipEventRecord(evJoin, st2), id);
-    TG_HIP(hipStreamWaitEvent(st, evJoin, 0), id);
     if (nd[m] > 0) {  // separator
         const rocblas_int nm = (rocblas_int)nd[m];
         if (m > 0 && nd[m - 1] > 0)
@@ -367,7 +358,7 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
         if (m + 1 < P && nd[m + 1] > 0)
             rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, nm, (rocblas_int)nd[m + 1], &mone,
                              As.p + asOff[m + 1], (rocblas_int)nd[m + 1], &one, Ad.p + adOff[m], nm), "rocblas_dsyrk");
-        factor_plane(m, h, st);
+        factor_plane(m);
     }
     // Solve operators, row-major with zero-padded rows (column-major results of transposed products):
     //   LT_p = inv(L_pp) (row-major copy); inv(L_pp)^T is the column-major Linv itself.
@@ -376,16 +367,12 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
     //            m: [inv(L_mm) L_{m,m-1} | inv(L_mm) L_{m,m+1}]
     //   backward A: inv(L_pp)^T L_{p+1,p}^T   = (As_{p+1} Linv_p)^T      (n_p x n_{p+1})
     //            B: inv(L_pp)^T L_{p-1,p}^T   = (X_p^T Linv_p)^T         (n_p x n_{p-1})
-    TG_HIP(hipEventRecord(evFork, st), id);
-    TG_HIP(hipStreamWaitEvent(st2, evFork, 0), id);
     for (int p = 0; p < P; ++p) {
         const rocblas_int np = (rocblas_int)nd[p];
         if (np == 0) continue;
         const int64_t prev = p > 0 ? nd[p - 1] : 0, next = p + 1 < P ? nd[p + 1] : 0;
         const double *Lp = Linv.p + adOff[p];
-        rocblas_handle h = p > m ? h2 : this->h;
-        hipStream_t ss = p > m ? st2 : st;
-        k_transpose_block<<<grid_for((int64_t)np * np, 256), 256, 0, ss>>>(Lp, LT.p + adOff[p], np);
+        k_transpose_block<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Lp, LT.p + adOff[p], np);
         TG_LAUNCH_CHECK(id);
         double *F = Fop.p + fOff[p], *B = Bop.p + bOff[p];
         const rocblas_int fl = (rocblas_int)fLd[p], bl = (rocblas_int)bLd[p];
@@ -403,16 +390,14 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
             rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)prev, np, np, &one,
                              As.p + asOff[p], np, Lp, np, &zero, B, bl), "rocblas_dgemm B B");
     }
-    TG_HIP(hipEventRecord(evJoin, st2), id);
-    TG_HIP(hipStreamWaitEvent(st, evJoin, 0), id);
-    std::vector<rocblas_int> hinfo(2 * P);
-    TG_HIP(hipMemcpyAsync(hinfo.data(), info.p, sizeof(rocblas_int) * 2 * P, hipMemcpyDeviceToHost, st), id);
+    std::vector<rocblas_int> hinfo(P);
+    TG_HIP(hipMemcpyAsync(hinfo.data(), info.p, sizeof(rocblas_int) * P, hipMemcpyDeviceToHost, st), id);
     TG_HIP(hipStreamSynchronize(st), id);
-    for (int p = 0; p < 2 * P; ++p)
+    for (int p = 0; p < P; ++p)
         if (hinfo[p] != 0)
             fail(TG_ERR_NUMERIC, "tg_solver:coarse",
                  "coarsest-level factorisation failed on plane block %d (info %d): matrix not positive definite",
-                 p % P, (int)hinfo[p]);
+                 p, (int)hinfo[p]);
     factored = true;
 }
 

@@ -49,8 +49,11 @@ class Model:
     def __del__(self):
         h = getattr(self, 'h', None)
         if h:
-            lib().tg_model_destroy(h)
             self.h = None
+            try:
+                lib().tg_model_destroy(h)
+            except Exception:   # interpreter shutdown: module globals already torn down
+                pass
 
     # ---- conditions ----
     def builtin_case(self, opt_case):

@@ -44,8 +44,11 @@ class Solver:
     def __del__(self):
         h = getattr(self, 'h', None)
         if h:
-            lib().tg_solver_destroy(h)
             self.h = None
+            try:
+                lib().tg_solver_destroy(h)
+            except Exception:   # interpreter shutdown: module globals already torn down
+                pass
 
     def ndof(self, level=1):
         return 3 * self.levels[level - 1]['numNodes']

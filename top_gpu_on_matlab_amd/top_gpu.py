@@ -43,8 +43,11 @@ class PDEFilter:
     def __del__(self):
         h = getattr(self, 'h', None)
         if h:
-            lib().tg_filter_destroy(h)
             self.h = None
+            try:
+                lib().tg_filter_destroy(h)
+            except Exception:   # interpreter shutdown: module globals already torn down
+                pass
 
     def __call__(self, src):
         src = src.contiguous()
@@ -66,8 +69,11 @@ class ConvolutionFilter:
     def __del__(self):
         h = getattr(self, 'h', None)
         if h:
-            lib().tg_convfilter_destroy(h)
             self.h = None
+            try:
+                lib().tg_convfilter_destroy(h)
+            except Exception:   # interpreter shutdown: module globals already torn down
+                pass
 
     def forward(self, x):
         out = torch.empty_like(x)

@@ -223,6 +223,11 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *                         super-element finest level): 4, 6, 8, 12 or 24 (default 24) */
 enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4 };
 int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value);
+/* Damped-Jacobi weight of the V-cycle smoother (the reference's global weightFactorJacobi_,
+ * TOP_GPU.m:397, 784, 809-811; default 0.6).  0 < omega < 2.  The smoother only reduces every error
+ * mode when omega * lambda_max(D^-1 K) < 2: on ragged non-cuboid domains lambda_max grows past
+ * 2 / 0.6 and PCG with the default weight stalls. */
+int tg_solver_set_jacobi_weight(tg_solver *s, double omega);
 /* Launch count and summed duration (ms) of the timed finest K*u launches since timing was enabled,
  * for precision 0 (FP64) / 1 (FP32) and epilogue 0 (K*u), 1 (r - K*u), 2 (damped Jacobi),
  * 3 (r - K*x with the pre-smoothing x = omega r ./ diag fused in). */

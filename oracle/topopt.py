@@ -123,12 +123,12 @@ class ConvolutionFilter:
 
 
 def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0, penal=3.0,
-            tol=1e-3, maxIT=800, verbose=False, x_init=None):
+            tol=1e-3, maxIT=800, verbose=False, x_init=None, weight_factor_jacobi=0.6):
     """Optimization driver TOP_GPU (TOP_GPU.m:93-292), FP64, no passive elements.
 
     Returns dict with cHist, volHist, sharpHist, consHist, lssIts, cSolid, xPhys, U, times.
     """
-    sv = S.Solver(model)
+    sv = S.Solver(model, weight_factor_jacobi=weight_factor_jacobi)
     lv1 = model.levels[0]
     Ne = lv1.numElements
     se = model.super_element

@@ -110,6 +110,8 @@ def test_solver_level_entry_points_reject_bad_arguments_without_a_device():
                               ctypes.byref(its))
     assert st == 1 and L.tg_last_error_id() == b'tg_compliance_grad:Input'
     assert L.tg_solver_set_tuning(None, 1, 0) == 1
+    assert L.tg_solver_set_jacobi_weight(None, 0.5) == 1
+    assert L.tg_last_error_id() == b'tg_solver_set_jacobi_weight:Input'
     n = ctypes.c_int64()
     assert L.tg_solver_kfine_timing(None, 0, 0, ctypes.byref(n), ctypes.byref(c)) == 1
     assert L.tg_last_error_id() == b'tg_solver_kfine_timing:Input'

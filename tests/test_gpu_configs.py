@@ -119,3 +119,67 @@ def test_mixed_precision_cantilever_96x192x96_against_fp64():
         xPhys = filt(x)
     print('\n[96x192x96 mixed vs fp64] rel compliance differences %s' % ['%.1e' % e for e in errs], flush=True)
     assert max(errs) < 1e-6
+
+
+@pytest.mark.parametrize('non_dyadic', [1, 0])
+def test_value_noise_pcg_with_smaller_jacobi_weight_matches_oracle(tmp_path, non_dyadic):
+    """The V-cycle smoother weight as a parameter (the reference's global weightFactorJacobi_): on the
+    64 x 32 x 32 value-noise box, omega = 0.4 instead of 0.6 (0.6 exceeds 2 / lambda_max(D^-1 K)
+    there) cuts the first solve from 105 to 24 PCG iterations (54 to 18 on the dyadic hierarchy); the
+    device follows the oracle with the same weight (same count, compliance within 20x the oracle's
+    unit-roundoff spread)."""
+    import torch
+    from oracle import solver as S
+    from top_gpu_on_matlab_amd.solver import Solver
+    from top_gpu_on_matlab_amd.synthetic import bottom_clamp_top_patch, value_noise_mask
+    from top_gpu_on_matlab_amd.topvoxel import load_topvoxel_model, read_topvoxel, write_topvoxel
+    mask = value_noise_mask(64, 32, 32, seed=0, octaves=4, period=32)
+    path = os.path.join(tmp_path, 'value_noise.TopVoxel')
+    write_topvoxel(path, mask, *bottom_clamp_top_patch(mask))
+    d = read_topvoxel(path)
+    om = M.build_file_model(d['nelx'], d['nely'], d['nelz'], d['voxels'], d['fixing'], d['loads'][0],
+                            non_dyadic=non_dyadic)
+    osv = S.Solver(om, weight_factor_jacobi=0.4)
+    osv.assemble(np.ones(om.levels[0].numElements))
+    b = om.F[:, 0]
+    y_ref, its_ref, _ = osv.pcg(b, 1e-3, 800)
+    bp = b * (1 + 2.0 ** -52 * np.random.default_rng(11).standard_normal(b.shape))
+    y_per, its_per, _ = osv.pcg(bp, 1e-3, 800)
+    c_ref = float(b @ y_ref)
+    spread = abs(float(b @ y_per) - c_ref) / c_ref
+    gm = load_topvoxel_model(path, non_dyadic=non_dyadic)
+    gsv = Solver(gm)
+    gsv.set_modulus(torch.ones(gm.num_elements, dtype=torch.float64, device='cuda'))
+    gsv.assemble()
+    its06 = gsv.pcg(b)[1]
+    gsv.set_jacobi_weight(0.4)
+    y, its, _ = gsv.pcg(b)
+    dev = abs(float(b @ y.cpu().numpy()) - c_ref) / c_ref
+    print('\n[value-noise non_dyadic=%d] PCG its: omega 0.6 -> %d, omega 0.4 -> %d (oracle %d, perturbed %d); '
+          'compliance %.1e (oracle spread %.1e)' % (non_dyadic, its06, its, its_ref, its_per, dev, spread), flush=True)
+    assert abs(its - its_ref) <= 3 * abs(its_per - its_ref)
+    assert dev < max(20 * spread, 1e-12)
+    assert 2 * its < its06
+
+
+def test_value_noise_full_size_256x128x128_converges():
+    """The full 256 x 128 x 128 value-noise workload (1,676,295 voxels, 5.67M DOFs, .TopVoxel file
+    branch): with the reference's weight 0.6 every PCG solve runs to maxIT = 800 at a relative
+    residual near 0.1; with weightFactorJacobi = 0.4 three optimisation iterations converge
+    (< 300 PCG iterations each), keep the volume at V0 and lower the compliance."""
+    import tempfile
+    from top_gpu_on_matlab_amd.synthetic import bottom_clamp_top_patch, value_noise_mask
+    from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
+    from top_gpu_on_matlab_amd.topvoxel import load_topvoxel_model, write_topvoxel
+    g = json.load(open(os.path.join(ROOT, 'tests', 'golden', 'value_noise_history.json')))
+    mask = value_noise_mask(256, 128, 128, seed=0, octaves=4, period=32)
+    assert hashlib.sha256(np.packbits(mask).tobytes()).hexdigest() == g['full_256x128x128']['mask_sha256']
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, 'value_noise.TopVoxel')
+        write_topvoxel(path, mask, *bottom_clamp_top_patch(mask))
+        gm = load_topvoxel_model(path)
+        got = TOP_GPU(path, V0=0.4, rMin=2.0, nLoop=3, ft=2, model=gm, weightFactorJacobi=0.4)
+    print('\n[value-noise 256x128x128, omega 0.4] its %s, c %s' % (got['lssIts'], got['cHist']), flush=True)
+    assert max(got['lssIts']) < 300
+    assert all(abs(v - 0.4) < 1e-4 for v in got['volHist'])
+    assert all(np.isfinite(got['cHist'])) and got['cHist'][2] < got['cHist'][1] < got['cHist'][0]

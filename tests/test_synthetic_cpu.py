@@ -67,3 +67,33 @@ def test_value_noise_masks_match_committed_hashes():
                              solid_fraction=c['solid_fraction'])
         assert int(m.sum()) == want['solid_voxels']
         assert hashlib.sha256(np.packbits(m).tobytes()).hexdigest() == want['mask_sha256']
+
+
+def test_value_noise_domain_overdamps_jacobi_at_the_reference_weight(tmp_path):
+    """Why PCG stalls on ragged domains with the reference's smoother weight (weightFactorJacobi_ =
+    0.6, TOP_GPU.m:397): a Rayleigh quotient of D^-1 K (a lower bound on its largest eigenvalue) on
+    the 48 x 24 x 24 value-noise domain (period 32, as the 256 x 128 x 128 workload) exceeds 2 / 0.6,
+    so I - 0.6 D^-1 K has an eigenvalue below -1: the smoother amplifies that mode and the V-cycle is
+    no longer a positive definite preconditioner.  With omega = 0.4 the oracle's PCG converges in a
+    fraction of the iterations."""
+    from oracle import solver as S
+    from top_gpu_on_matlab_amd.synthetic import bottom_clamp_top_patch, value_noise_mask
+    from top_gpu_on_matlab_amd.topvoxel import read_topvoxel, write_topvoxel
+    mask = value_noise_mask(48, 24, 24, seed=0, octaves=4, period=32)
+    path = os.path.join(tmp_path, 'value_noise.TopVoxel')
+    write_topvoxel(path, mask, *bottom_clamp_top_patch(mask))
+    d = read_topvoxel(path)
+    om = M.build_file_model(d['nelx'], d['nely'], d['nelz'], d['voxels'], d['fixing'], d['loads'][0])
+    its = {}
+    for w in (0.6, 0.4):
+        sv = S.Solver(om, weight_factor_jacobi=w)
+        sv.assemble(np.ones(om.levels[0].numElements))
+        its[w] = sv.pcg(om.F[:, 0], 1e-3, 800)[1]
+    dg = om.levels[0].diagK
+    v = np.random.default_rng(0).standard_normal(dg.size)
+    for _ in range(40):
+        w = sv.kbyu(v, 1) / dg
+        v = w / np.linalg.norm(w)
+    rq = float(v @ sv.kbyu(v, 1)) / float(v @ (dg * v))
+    assert 0.6 * rq > 2.0 > 0.4 * rq, rq
+    assert its[0.4] * 2 < its[0.6], its

@@ -87,6 +87,7 @@ SIGNATURES = {
     'tg_solver_last_timing': (ctypes.c_int, [_VP, ctypes.POINTER(_D)]),
     'tg_solver_time_kbyu': (ctypes.c_int, [_VP, _I32, _I32, _I32, ctypes.POINTER(_D)]),
     'tg_solver_set_tuning': (ctypes.c_int, [_VP, _I32, _I32]),
+    'tg_solver_set_jacobi_weight': (ctypes.c_int, [_VP, _D]),
     'tg_solver_kfine_timing': (ctypes.c_int, [_VP, _I32, _I32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_D)]),
     'tg_solver_unit_compliance': (ctypes.c_int, [_VP, _VP, _VP]),
     'tg_solver_subvoxel_energy': (ctypes.c_int, [_VP, _VP, _VP]),

@@ -940,6 +940,15 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
     });
 }
 
+int tg_solver_set_jacobi_weight(tg_solver *s, double omega) {
+    return guard("tg_solver_set_jacobi_weight:Input", [&] {
+        const char *id = "tg_solver_set_jacobi_weight:Input";
+        Solver &S = S_of(s, id);
+        if (!(omega > 0.0 && omega < 2.0)) fail(TG_ERR_ARGS, id, "omega must lie in (0, 2), got %g", omega);
+        S.omega = omega;
+    });
+}
+
 int tg_solver_kfine_timing(tg_solver *s, int32_t precision, int32_t epilogue, int64_t *launches, double *ms_total) {
     return guard("tg_solver_kfine_timing:Input", [&] {
         const char *id = "tg_solver_kfine_timing:Input";

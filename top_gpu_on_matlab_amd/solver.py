@@ -132,6 +132,10 @@ class Solver:
         if kfine_zchunk is not None:
             check(lib().tg_solver_set_tuning(self.h, TG_TUNE_KFINE_ZCHUNK, int(kfine_zchunk)))
 
+    def set_jacobi_weight(self, omega):
+        """Damped-Jacobi weight of the V-cycle smoother (weightFactorJacobi_, TOP_GPU.m:397; 0.6)."""
+        check(lib().tg_solver_set_jacobi_weight(self.h, float(omega)))
+
     def time_kfine(self, on=True):
         """Start (clearing the record) or stop bracketing each finest-level K*u launch with HIP events."""
         from ._lib import TG_TUNE_TIME_KFINE

@@ -134,11 +134,13 @@ def subvoxel_energy(solver, U):
 
 def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, mixed_Precision=0,
             super_element=0, *, Emin=1e-6, E0=1.0, penal=3.0, nu=0.3, tol=1e-3, maxIT=800, x0=None,
-            out_path=None, verbose=False, model=None, tuning=None):
+            out_path=None, verbose=False, model=None, tuning=None, weightFactorJacobi=0.6):
     """TOP_GPU(inputModel, 'optCase', ..., 'V0', ..., 'rMin', ..., 'nLoop', ..., 'ft', ...,
     'mixed_Precision', ..., 'super_element', ...)  (TOP_GPU.m:1-292).
 
-    inputModel: logical voxel array [nely, nelx, nelz] or a .TopVoxel path.  Returns a dict with
+    inputModel: logical voxel array [nely, nelx, nelz] or a .TopVoxel path.  weightFactorJacobi is
+    the V-cycle's damped-Jacobi weight (the reference's global of that name, 0.6); ragged domains
+    need a smaller one (PCG stalls when it exceeds 2 / lambda_max(D^-1 K)).  Returns a dict with
     the histories (cHist, volHist, sharpHist, consHist, lssIts, tHist), cSolid and xPhys."""
     num = 4
     se = bool(super_element)
@@ -161,6 +163,8 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
     solver = Solver(model)
     if tuning:
         solver.set_tuning(**tuning)
+    if weightFactorJacobi != 0.6:
+        solver.set_jacobi_weight(weightFactorJacobi)
     Ne = model.num_elements
     dev = 'cuda'
     if se:

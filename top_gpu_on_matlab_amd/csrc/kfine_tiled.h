@@ -137,7 +137,7 @@ __device__ __forceinline__ void walsh_inv(T x[8]) {
 // EPI_RESID_JAC0: the V-cycle's pre-smoothing fused in: the operand is x = omega r ./ diag (formed
 // while staging; u is not read), x is written to xout and out = r - K x.
 template <typename T, int EPI, bool CK, bool DOT>
-__global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) void k_kfine_walsh(const T *__restrict__ u, T *__restrict__ out,
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(const T *__restrict__ u, T *__restrict__ out,
                                                      const T *__restrict__ E, const uint8_t *__restrict__ mask,
                                                      const KWVals<T> kw, T cs, const Grid g, const KTiles kt,
                                                      const T *__restrict__ r, const T *__restrict__ diag, T omega,
@@ -161,38 +161,46 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
     const int64_t NYX = (int64_t)g.NY * g.NX;
     const int64_t EYX = (int64_t)g.EY * g.EX;
 
+    // Addressing: every load is unconditional, at a 32-bit lane offset inside a node (element)
+    // plane from a plane base that is uniform across the workgroup; coordinates outside the grid
+    // are clamped and the loaded value is masked when it is committed.  Straight-line loads keep
+    // the memory-counter waits static: a wait for one group of loads never drains a younger one.
     // element lane
     const int ly = t % KT_EY, lx = t / KT_EY;
     const int ey = y0 - 1 + ly, ex = x0 - 1 + lx;
     const bool eIn = (unsigned)ey < (unsigned)g.EY && (unsigned)ex < (unsigned)g.EX;
-    const T *Ecol = E + (eIn ? ey + (int64_t)g.EY * ex : 0);
+    const int offE = min(max(ey, 0), g.EY - 1) + g.EY * min(max(ex, 0), g.EX - 1);
     // node lane (lanes >= KT_TY * KT_TX own none)
     const int oy = t % KT_TY, ox = t / KT_TY;
     const bool owner = t < KT_TY * KT_TX && y0 + oy < g.NY && x0 + ox < g.NX;
+    const int offO = 3 * (min(y0 + oy, g.NY - 1) + g.NY * min(x0 + ox, g.NX - 1));
+    // staged values of the halo plane: value i = t + 256 k is component i % 3 of node i / 3
+    int offU[KT_NI];
+    bool okXY[KT_NI];
+#pragma unroll
+    for (int k = 0; k < KT_NI; ++k) {
+        const int i = min(t + 256 * k, 3 * KT_PN - 1);
+        const int node = i / 3, c = i - 3 * node;
+        const int gy = y0 - 1 + node % KT_PY, gx = x0 - 1 + node / KT_PY;
+        okXY[k] = (unsigned)gy < (unsigned)g.NY && (unsigned)gx < (unsigned)g.NX;
+        offU[k] = 3 * (min(max(gy, 0), g.NY - 1) + g.NY * min(max(gx, 0), g.NX - 1)) + c;
+    }
+    auto zclamp = [&](int z, int n) { return min(max(z, 0), n - 1); };
 
-    // Prefetch: loads are unconditional (clamped addresses) and all masking is applied at commit
-    // time, so no wait for the loaded values is needed until the next layer.
+    // next plane's staged values (registers until committed)
     T pre[KT_NI], preD[J0 ? KT_NI : 1];
     uint8_t preM[KT_NI];
-    bool preOk[KT_NI];
+    bool preZ = false;
     auto fetch = [&](int z) {
+        const int64_t b = NYX * zclamp(z, g.NZ);
+        const T *src = (J0 ? r : u) + 3 * b;
 #pragma unroll
         for (int k = 0; k < KT_NI; ++k) {
-            const int i = min(t + 256 * k, 3 * KT_PN - 1);
-            const int node = i / 3, c = i - 3 * node;
-            const int py = node % KT_PY, px = node / KT_PY;
-            const int gy = y0 - 1 + py, gx = x0 - 1 + px;
-            const bool ok = (unsigned)gy < (unsigned)g.NY && (unsigned)gx < (unsigned)g.NX && (unsigned)z < (unsigned)g.NZ;
-            const int64_t n = ok ? gy + (int64_t)g.NY * gx + NYX * z : 0;
-            if (J0) {
-                pre[k] = r[3 * n + c];
-                preD[J0 ? k : 0] = diag[3 * n + c];
-            } else {
-                pre[k] = u[3 * n + c];
-            }
-            preM[k] = mask[n];
-            preOk[k] = ok;
+            pre[k] = src[offU[k]];
+            if (J0) preD[J0 ? k : 0] = diag[3 * b + offU[k]];
+            preM[k] = mask[b + offU[k] / 3];
         }
+        preZ = (unsigned)z < (unsigned)g.NZ;
     };
     auto commit = [&](int slot) {
 #pragma unroll
@@ -200,9 +208,10 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
             const int i = t + 256 * k;
             if (i < 3 * KT_PN) {
                 const int node = i / 3, c = i - 3 * node;
-                const uint8_t mk = preOk[k] ? preM[k] : (uint8_t)0;
+                const bool ok = okXY[k] && preZ;
+                const uint8_t mk = ok ? preM[k] : (uint8_t)0;
                 const T v = J0 ? omega * (pre[k] / preD[J0 ? k : 0]) : pre[k];
-                sU[slot][c][node] = (preOk[k] && !(mk & (1 << c))) ? v : (T)0;
+                sU[slot][c][node] = (ok && !(mk & (1 << c))) ? v : (T)0;
                 if (c == 0) sM[slot][node] = mk;
             }
         }
@@ -210,14 +219,30 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
     T Eraw = 0;
     bool Eok = false;
     auto fetchE = [&](int layer) {
+        Eraw = E[EYX * zclamp(layer, g.EZ) + offE];
         Eok = eIn && (unsigned)layer < (unsigned)g.EZ;
-        Eraw = Ecol[Eok ? EYX * layer : 0];
+    };
+    // the owned node's own values for the epilogue (raw u: identity rows and the dot; r; diag),
+    // loaded one plane ahead, after this plane's stores
+    constexpr bool NEED_U = EPI == EPI_KU || EPI == EPI_RESID || EPI == EPI_JACOBI;
+    constexpr bool NEED_R = EPI != EPI_KU;
+    constexpr bool NEED_D = EPI == EPI_JACOBI || J0;
+    T oU[3] = {0, 0, 0}, oR[3] = {0, 0, 0}, oD[3] = {1, 1, 1};
+    auto fetchO = [&](int z) {
+        const int64_t b = 3 * NYX * zclamp(z, g.NZ) + offO;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (NEED_U) oU[d] = u[b + d];
+            if (NEED_R) oR[d] = r[b + d];
+            if (NEED_D) oD[d] = diag[b + d];
+        }
     };
 
     fetch(z0 - 1);
     commit(0);
     fetch(z0);
     fetchE(z0 - 1);
+    fetchO(z0 - 1);
     T part[3] = {0, 0, 0};  // upper-vertex contributions to the owned node of the next plane
     double dsum = 0;        // DOT: this lane's share, accumulated plane by plane
     int cur = 0;
@@ -227,10 +252,8 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
         lds_barrier();
         // the owned node's mask (slot cur is overwritten once every lane has passed the next barrier)
         const uint8_t mk = owner ? sM[cur][(oy + 1) + KT_PY * (ox + 1)] : (uint8_t)0;
-        if (ez + 1 < z1) {
-            fetch(ez + 2);
-            fetchE(ez + 1);
-        }
+        fetch(ez + 2);  // past the chunk on the last layer: clamped, never committed
+        fetchE(ez + 1);
         // ---- element vector of this lane's element
         T fv[24];
         {
@@ -288,33 +311,33 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
                     }
                 }
             if (ez >= z0) {
-                const int64_t n = (y0 + oy) + (int64_t)g.NY * (x0 + ox) + NYX * ez;
+                T *o = out + 3 * NYX * ez + offO;
                 if (J0 && (mk & NODE_ACTIVE)) {
+                    T *xo = xout + 3 * NYX * ez + offO;
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
-                        const T rv = r[3 * n + d];
-                        const T xv = omega * (rv / diag[3 * n + d]);
-                        xout[3 * n + d] = xv;
-                        out[3 * n + d] = rv - ((mk & (1 << d)) ? xv : lo[d]);
+                        const T xv = omega * (oR[d] / oD[d]);
+                        xo[d] = xv;
+                        o[d] = oR[d] - ((mk & (1 << d)) ? xv : lo[d]);
                     }
                 } else if (mk & NODE_ACTIVE) {
 #pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        if (mk & (1 << d)) lo[d] = u[3 * n + d];
-                    epilogue<T, EPI>(out, n, lo, r, diag, u, omega);
-                    if (DOT) {
-                        const T *w = EPI == EPI_KU ? u : r;
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) dsum += (double)out[3 * n + d] * (double)w[3 * n + d];
+                    for (int d = 0; d < 3; ++d) {
+                        T v = (mk & (1 << d)) ? oU[d] : lo[d];
+                        if (EPI == EPI_RESID) v = oR[d] - v;
+                        if (EPI == EPI_JACOBI) v = oU[d] + (omega * (oR[d] - v)) / oD[d];
+                        o[d] = v;
+                        if (DOT) dsum += (double)v * (double)(EPI == EPI_KU ? oU[d] : oR[d]);
                     }
                 } else {
-                    out[3 * n] = 0;
-                    out[3 * n + 1] = 0;
-                    out[3 * n + 2] = 0;
+                    o[0] = 0;
+                    o[1] = 0;
+                    o[2] = 0;
                     if (J0) {
-                        xout[3 * n] = 0;
-                        xout[3 * n + 1] = 0;
-                        xout[3 * n + 2] = 0;
+                        T *xo = xout + 3 * NYX * ez + offO;
+                        xo[0] = 0;
+                        xo[1] = 0;
+                        xo[2] = 0;
                     }
                 }
             }
@@ -339,6 +362,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 && EPI != EPI_KU) ? 3 : 4) voi
         part[0] = up[0];
         part[1] = up[1];
         part[2] = up[2];
+        fetchO(ez + 1);
         cur ^= 1;
     }
     if (DOT) {

@@ -28,10 +28,10 @@ import numpy as np
 METRIC = 'PCG solves/sec + K·u HBM GB/s, mixed vs FP64 at 1e-6 rel compliance err'
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per launch of the FP64 finest K*u at 128x256x128 (bytes), from two rocprofv3 --pmc runs:
-# FETCH_SIZE 157,400 KiB x 2.0 + WRITE_SIZE 117,390 KiB x 1.0, the factors being what the counters
-# read for a plain 2 GiB device copy (FETCH 1,048,576 KiB, WRITE 2,097,152 KiB)
-KFINE64_TRAFFIC = 442.6e6
+# HBM bytes per launch of the FP64 finest K*u at 128x256x128 (bytes), from two rocprofv3 --pmc runs of
+# `bench.py --traffic-probe`: FETCH_SIZE 187,350 KiB x 2.0 + WRITE_SIZE 106,630 KiB x 1.0, the factors
+# being what the counters read for a plain 2 GiB device copy (FETCH 1,048,576 KiB, WRITE 2,097,152 KiB)
+KFINE64_TRAFFIC = 492.9e6
 
 
 def log(*a):
@@ -137,6 +137,34 @@ def cpu_baseline(shape, E, F, its_ref, sample_iters):
     }
 
 
+def traffic_probe(shape):
+    """Finest K*u launches at `shape` (moduli 1e-6 + U(0,1)^3, seed 0) and two 2 GiB device copies
+    (the runtime blit and a plain elementwise load/store kernel): under rocprofv3 --pmc the copies,
+    which move exactly 2 GiB in and 2 GiB out, give the factors that turn FETCH_SIZE and WRITE_SIZE
+    into HBM bytes."""
+    import torch
+    from top_gpu_on_matlab_amd.model import cuboid_model
+    from top_gpu_on_matlab_amd.solver import Solver
+    m = cuboid_model(*shape, 1, precision='mixed')
+    s = Solver(m)
+    g = torch.Generator().manual_seed(0)
+    s.set_modulus((1e-6 + torch.rand(m.num_elements, generator=g, dtype=torch.float64) ** 3).cuda())
+    s.assemble()
+    torch.cuda.synchronize()
+    log('finest K*u: fp64 %.1f us, fp32 %.1f us' % (s.time_kbyu(1, 'fp64', 3) * 1e3, s.time_kbyu(1, 'fp32', 3) * 1e3))
+    del s, m
+    torch.cuda.empty_cache()
+    a = torch.empty(256 * 1024 * 1024, dtype=torch.float64, device='cuda')
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(2):
+        b.copy_(a)
+    for _ in range(2):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    log('copies done (2 GiB read + 2 GiB write each)')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -145,11 +173,17 @@ def main():
     ap.add_argument('--size', type=int, nargs=3, default=[128, 256, 128], metavar=('NELY', 'NELX', 'NELZ'))
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--cpu-iters', type=int, default=1)
+    ap.add_argument('--traffic-probe', action='store_true',
+                    help='only run 3 + 3 finest K*u launches (FP64, FP32) and the 2 GiB copies that calibrate '
+                         'the FETCH_SIZE / WRITE_SIZE counters; for rocprofv3 --pmc runs')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: the hot path has no multi-GPU mode in this build '
               '(requested --gpus %d)' % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.traffic_probe:
+        traffic_probe(tuple(args.size))
+        return
     import torch
     shape = tuple(args.size)
     rng = np.random.default_rng(12345)

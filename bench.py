@@ -195,9 +195,11 @@ def main():
     log('setup + solid solves %.1fs (Ne=%d)' % (time.perf_counter() - t, run.Ne))
     hist = []
     for k in range(args.warmup + args.steps):
-        if k == args.warmup:  # per-launch HIP-event timing of the finest K*u inside the timed steps
-            for sv in run.solvers.values():
-                sv.time_kfine(True)
+        if k == args.warmup:
+            # per-launch HIP-event timing, inside the timed steps, of the roofline kernel: the FP64
+            # finest K*u that is the PCG operator, as the mixed solver launches it (the events cost
+            # ~7 us per launch, ~1% of its PCG time, which `value` includes); the FP64 solver runs bare
+            run.solvers['mixed'].time_kfine(True, only_operator=True)
         r = run.iterate()
         hist.append(r)
         log('it %d: fp64 PCG %.1f ms (%d its) c=%.12e | mixed PCG %.1f ms (%d its) c=%.12e | rel %.1e | '
@@ -219,11 +221,8 @@ def main():
     # and smoothing applies
     Ne, Nn = run.Ne, run.model.num_nodes
     kst = {}
-    for name, p, prec, epi in (('fp64_ku', 'fp64', 'fp64', 'ku'), ('fp64_resid_jac0', 'fp64', 'fp64', 'resid_jac0'),
-                               ('fp64_jacobi', 'fp64', 'fp64', 'jacobi'), ('fp32_resid_jac0', 'mixed', 'fp32', 'resid_jac0'),
-                               ('fp32_jacobi', 'mixed', 'fp32', 'jacobi')):
-        n, ms = run.solvers[p].kfine_timing(prec, epi)
-        kst[name] = dict(launches=n, avg_us=1e3 * ms / max(n, 1))
+    n, ms = run.solvers['mixed'].kfine_timing('fp64', 'ku')
+    kst['fp64_ku'] = dict(launches=n, avg_us=1e3 * ms / max(n, 1))
     b64, b32 = kbyu_alg_bytes(Nn, Ne, 8, 8, 8), kbyu_alg_bytes(Nn, Ne, 4, 4, 4)
     us64 = kst['fp64_ku']['avg_us']
     gbs64 = b64 / us64 / 1e3
@@ -263,7 +262,8 @@ def main():
         'mixed_over_fp64': res['mixed']['solves_per_s'] / res['fp64']['solves_per_s'],
         'rel_compliance_err_mixed_vs_fp64': {'max': max(errs), 'per_iteration': errs},
         'kbyu': kst,
-        'roofline': {'kernel': 'finest-level K*u, FP64, PCG operator (k_kfine_walsh<double,0,true>)', 'bound': 'hbm',
+        'roofline': {'kernel': 'finest-level K*u, FP64, PCG operator with the fused p.Kp (k_kfine_walsh<double,0,true,true>), '
+                               'launches of the mixed solver in the timed steps', 'bound': 'hbm',
                      'achieved': gbs64, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': gbs64 / HBM_PEAK_GBS,
                      'traffic': KFINE64_TRAFFIC, 'bytes_alg': b64, 'avg_launch_us': us64, 'launches': kst['fp64_ku']['launches'],
                      'dense_fp64_flop_frac': tf64 / FP64_PEAK_TFLOPS},

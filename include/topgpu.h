@@ -217,8 +217,8 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  * K*u is a MATLAB pagemtimes, TOP_GPU.m:1729-1741).  Keys:
  *   TG_TUNE_KFINE_KERNEL  0 = element-centric LDS-tiled z-march (default), 1 = node-centric gather
  *   TG_TUNE_KFINE_ZCHUNK  node planes per workgroup of the tiled kernel (1..4096, default 12)
- *   TG_TUNE_TIME_KFINE    1 = bracket every tiled finest K*u launch with HIP events (and clear the
- *                         record), 0 = stop
+ *   TG_TUNE_TIME_KFINE    1 = bracket every tiled finest K*u launch with HIP events, 2 = only the
+ *                         plain products (the PCG operator), 0 = stop; 1 and 2 clear the record
  *   TG_TUNE_KELEM_PARTS   threads per element of the per-element-matrix K*u (coarse levels and the
  *                         super-element finest level): 4, 6, 8, 12 or 24 (default 24) */
 enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4 };

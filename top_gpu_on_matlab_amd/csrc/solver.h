@@ -44,7 +44,7 @@ struct Solver {
         hipEvent_t ev[2] = {nullptr, nullptr};
         int key = 0;  // 4 * (FP32 ? 1 : 0) + epilogue
     };
-    bool time_fine = false;
+    int time_fine = 0;  // 1: every tiled finest K*u launch, 2: the plain products (EPI_KU) only
     std::vector<TimedLaunch> tpool;
     size_t tused = 0;
     hipEvent_t *timing_slot(int key);

@@ -526,7 +526,7 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
         if (kfine_kernel == 0 && walsh_ok) {
             const KTiles kt = ktiles_for(D.g, kfine_zchunk);
             const unsigned gt = (unsigned)kt.count();
-            hipEvent_t *ev = time_fine ? timing_slot((f64 ? 0 : 4) + epi) : nullptr;
+            hipEvent_t *ev = (time_fine == 1 || (time_fine == 2 && epi == EPI_KU)) ? timing_slot((f64 ? 0 : 4) + epi) : nullptr;
             double *dp = nullptr;
             if (dot_slot >= 0 && epi != EPI_RESID) {
                 if ((int64_t)dotp.n < (int64_t)gt) dotp.alloc(gt);
@@ -933,8 +933,8 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
         else if (key == TG_TUNE_KFINE_ZCHUNK && value >= 1 && value <= 4096) S.kfine_zchunk = value;
         else if (key == TG_TUNE_KELEM_PARTS && (value == 4 || value == 6 || value == 8 || value == 12 || value == 24))
             S.kelem_parts = value;
-        else if (key == TG_TUNE_TIME_KFINE && (value == 0 || value == 1)) {
-            S.time_fine = value != 0;
+        else if (key == TG_TUNE_TIME_KFINE && value >= 0 && value <= 2) {
+            S.time_fine = value;
             S.tused = 0;
         } else fail(TG_ERR_ARGS, id, "unknown tuning key or value out of range");
     });

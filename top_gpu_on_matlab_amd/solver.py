@@ -136,10 +136,11 @@ class Solver:
         """Damped-Jacobi weight of the V-cycle smoother (weightFactorJacobi_, TOP_GPU.m:397; 0.6)."""
         check(lib().tg_solver_set_jacobi_weight(self.h, float(omega)))
 
-    def time_kfine(self, on=True):
-        """Start (clearing the record) or stop bracketing each finest-level K*u launch with HIP events."""
+    def time_kfine(self, on=True, only_operator=False):
+        """Start (clearing the record) or stop bracketing each finest-level K*u launch with HIP events
+        (only_operator: just the plain products K*u, i.e. the PCG operator, not the V-cycle's)."""
         from ._lib import TG_TUNE_TIME_KFINE
-        check(lib().tg_solver_set_tuning(self.h, TG_TUNE_TIME_KFINE, 1 if on else 0))
+        check(lib().tg_solver_set_tuning(self.h, TG_TUNE_TIME_KFINE, (2 if only_operator else 1) if on else 0))
 
     def kfine_timing(self, precision='fp64', epilogue='ku'):
         """(launches, total ms) of the timed finest K*u launches; epilogue 'ku', 'resid', 'jacobi' or

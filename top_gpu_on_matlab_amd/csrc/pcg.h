@@ -12,12 +12,15 @@ namespace tg {
 
 enum { S_PV = 0, S_RR = 1, S_BB = 2, S_XA = 3, S_XB = 4 };
 
-// y += lambda p; r -= lambda v; partial sums of r.r  (TOP_GPU.m:630-632, 637)
+// y += lambda p; r -= lambda v; partial sums of r.r  (TOP_GPU.m:630-632, 637).  R32: also store r
+// rounded to FP32 (the mixed V-cycle's input, TOP_GPU.m:775-777), saving a conversion pass.
+template <bool R32>
 __global__ __launch_bounds__(RED_THREADS) static void k_pcg_update(double *__restrict__ y, double *__restrict__ r,
                                                                    const double *__restrict__ p,
                                                                    const double *__restrict__ v, int64_t n,
                                                                    const double *__restrict__ scal, int ix1, int ipv,
-                                                                   double *__restrict__ partial) {
+                                                                   double *__restrict__ partial,
+                                                                   float *__restrict__ r32) {
     const double lam = scal[ix1] / scal[ipv];
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
@@ -26,6 +29,7 @@ __global__ __launch_bounds__(RED_THREADS) static void k_pcg_update(double *__res
         y[i] = y[i] + lam * p[i];
         const double rr = r[i] - lam * v[i];
         r[i] = rr;
+        if (R32) r32[i] = (float)rr;
         s += rr * rr;
     }
     s = block_sum<double>(s);
@@ -66,6 +70,8 @@ struct PcgWork {
     hipStream_t st = nullptr;
     DevBuf<double> r, z, p, v, partial, scal;
     double *h_scal = nullptr;
+    float *r32 = nullptr;     // optional FP32 copy of r kept by the update (mixed V-cycle input)
+    bool r32_fresh = false;   // r32 holds the current r
 
     void init(int64_t count, hipStream_t s) {
         n = count;
@@ -109,13 +115,19 @@ struct PcgWork {
             TG_HIP(hipMemcpyAsync(r.p, b, n * sizeof(double), hipMemcpyDeviceToDevice, st), id);
         }
         int sx1 = S_XA, sx2 = S_XB;
+        r32_fresh = false;
         if (!ptv(r.p, z.p, sx1)) dot(z, r, sx1);
         TG_HIP(hipMemcpyAsync(p.p, z.p, n * sizeof(double), hipMemcpyDeviceToDevice, st), id);
         int its = 0;
         double res = 0;
         for (its = 1; its <= maxIT; ++its) {
             if (!atx(p.p, v.p, S_PV)) dot(p, v, S_PV);
-            k_pcg_update<<<RED_BLOCKS, RED_THREADS, 0, st>>>(y, r, p, v, n, scal, sx1, S_PV, partial);
+            if (r32) {
+                k_pcg_update<true><<<RED_BLOCKS, RED_THREADS, 0, st>>>(y, r, p, v, n, scal, sx1, S_PV, partial, r32);
+                r32_fresh = true;
+            } else {
+                k_pcg_update<false><<<RED_BLOCKS, RED_THREADS, 0, st>>>(y, r, p, v, n, scal, sx1, S_PV, partial, nullptr);
+            }
             k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR);
             TG_LAUNCH_CHECK(id);
             res = std::sqrt(read(S_RR)) / normB;

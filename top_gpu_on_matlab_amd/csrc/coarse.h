@@ -6,14 +6,15 @@
 //   chain A, p = 0 .. m-1:     L_pp L_pp^T = A_pp - L_{p,p-1} L_{p,p-1}^T,  L_{p+1,p} = A_{p+1,p} L_pp^-T
 //   chain B, p = P-1 .. m+1:   L_pp L_pp^T = A_pp - L_{p,p+1} L_{p,p+1}^T,  L_{p-1,p} = A_{p-1,p} L_pp^-T
 //   separator:                 L_mm L_mm^T = A_mm - L_{m,m-1} L_{m,m-1}^T - L_{m,m+1} L_{m,m+1}^T
-// on the device (rocSOLVER potrf, rocBLAS trsm/syrk/gemm, all on the solver stream).  A solve is a batched triangular
+// on the device (blocked Cholesky: LDS kernels for the diagonal blocks and panels, rocBLAS syrk for
+// the trailing updates; rocBLAS trsm/gemm for the rest; all on the solver stream).  A solve is a batched triangular
 // product y = blockdiag(inv(L_pp)) b, the two forward chains advanced together (one launch per step,
 // ~P/2 steps) and the separator, then x = blockdiag(inv(L_pp)^T) y and the two backward chains
 // outwards from the separator; the whole sequence is captured once as a HIP graph (its buffers are
 // fixed at setup) and replayed on every solve.
 #pragma once
 
-#include <rocsolver/rocsolver.h>
+#include <rocblas/rocblas.h>
 
 #include "kernels.h"
 #include "model.h"
@@ -37,6 +38,7 @@ struct CoarseSolver {
     DevBuf<double> Ad, As, Linv, LT, Fop, Bop;
     DevBuf<double> bb, yy, xx;
     DevBuf<rocblas_int> info;
+    DevBuf<double> chW;             // inv(L_kk) of the current Cholesky panel
     rocblas_handle h = nullptr;
     bool factored = false;
     hipGraphExec_t exec = nullptr;  // the captured solve sequence

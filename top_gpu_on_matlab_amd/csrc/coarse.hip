@@ -50,6 +50,114 @@ __global__ void k_coarse_assemble(const double *__restrict__ Ks, int64_t ldk, co
     else As[asOff[pJ] + row + np * col] = val;
 }
 
+// ---- plane-block Cholesky: blocked right-looking, panels of CH_NB columns --------------------
+// Per panel: the kb x kb diagonal block is factorised in LDS by one workgroup, which also forms
+// W = inv(L_kk); the rows under it are X = P W^T (k_chol_panel); the trailing matrix is updated by a
+// rocBLAS syrk.  Column-major, lower triangle; the upper triangle is not referenced.
+constexpr int CH_NB = 64;
+constexpr int CH_PR = 32;  // panel rows per workgroup
+
+// info: set to col0 + j + 1 (LAPACK convention, first failure only) when pivot j is not positive;
+// the factorisation then continues with a unit pivot so every value stays finite.
+__global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64_t lda, int kb,
+                                                   double *__restrict__ W, int *__restrict__ info, int col0) {
+    __shared__ double a[CH_NB][CH_NB + 1];  // a[i][j] = A(i, j); the diagonal is kept in dg
+    __shared__ double w[CH_NB][CH_NB + 1];  // inv(L_kk)
+    __shared__ double dg[CH_NB];
+    const int t = threadIdx.x;
+    for (int idx = t; idx < kb * kb; idx += 256) {
+        const int i = idx % kb, j = idx / kb;
+        a[i][j] = A[i + (int64_t)j * lda];
+    }
+    __syncthreads();
+    int bad = 0;
+    const int ri = t % CH_NB, cph = t / CH_NB;  // trailing update: row ri, columns cph, cph + 4, ...
+    for (int j = 0; j < kb; ++j) {
+        double d = a[j][j];  // final since the previous step's update
+        if (!(d > 0.0)) {
+            if (!bad) bad = j + 1;
+            d = 1.0;
+        }
+        const double djj = sqrt(d);
+        if (t == 0) dg[j] = djj;
+        if (t > j && t < kb) a[t][j] /= djj;  // kb <= 64 < 256
+        __syncthreads();
+        // a(i, l) -= a(i, j) a(l, j) for j < l <= i < kb
+        if (ri > j && ri < kb) {
+            const double aij = a[ri][j];
+            int l = j + 1 + cph;
+            for (; l + 12 <= ri; l += 16) {  // four independent updates per trip: loads first
+                const double c0 = a[l][j], c1 = a[l + 4][j], c2 = a[l + 8][j], c3 = a[l + 12][j];
+                const double v0 = a[ri][l], v1 = a[ri][l + 4], v2 = a[ri][l + 8], v3 = a[ri][l + 12];
+                a[ri][l] = v0 - aij * c0;
+                a[ri][l + 4] = v1 - aij * c1;
+                a[ri][l + 8] = v2 - aij * c2;
+                a[ri][l + 12] = v3 - aij * c3;
+            }
+            for (; l <= ri; l += 4) a[ri][l] -= aij * a[l][j];
+        }
+        __syncthreads();
+    }
+    // W = inv(L_kk) row by row; four lanes per column split each dot product
+    const int wc = t / 4, part = t % 4;
+    for (int i = 0; i < kb; ++i) {
+        double acc = 0.0, acc2 = 0.0;
+        if (wc < i) {
+            int k = wc + part;
+            for (; k + 4 < i; k += 8) {  // two chains, loads first
+                const double p0 = a[i][k], q0 = w[k][wc], p1 = a[i][k + 4], q1 = w[k + 4][wc];
+                acc += p0 * q0;
+                acc2 += p1 * q1;
+            }
+            if (k < i) acc += a[i][k] * w[k][wc];
+        }
+        acc += acc2;
+        acc += __shfl_xor(acc, 1, 4);
+        acc += __shfl_xor(acc, 2, 4);
+        if (part == 0 && wc < kb) w[i][wc] = wc == i ? 1.0 / dg[i] : (wc < i ? -acc / dg[i] : 0.0);
+        __syncthreads();
+    }
+    for (int idx = t; idx < kb * kb; idx += 256) {
+        const int i = idx % kb, j = idx / kb;
+        if (i > j) A[i + (int64_t)j * lda] = a[i][j];
+        else if (i == j) A[i + (int64_t)j * lda] = dg[i];
+        W[i + CH_NB * j] = w[i][j];
+    }
+    if (t == 0 && bad && *info == 0) *info = col0 + bad;
+}
+
+// Rows under the diagonal block: X = P W^T for the m x kb panel P (column-major, lda), in place;
+// CH_PR rows per workgroup, lane (row r, column group cg) forms 8 outputs.
+__global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ P, int64_t lda, int m, int kb,
+                                                    const double *__restrict__ W) {
+    __shared__ double w[CH_NB][CH_NB + 1];
+    __shared__ double pr[CH_PR][CH_NB + 1];
+    const int t = threadIdx.x;
+    const int row0 = blockIdx.x * CH_PR;
+    const int rows = min(CH_PR, m - row0);
+    for (int idx = t; idx < kb * kb; idx += 256) {
+        const int i = idx % kb, j = idx / kb;
+        w[i][j] = W[i + CH_NB * j];
+    }
+    for (int idx = t; idx < rows * kb; idx += 256) {
+        const int r = idx % rows, l = idx / rows;
+        pr[r][l] = P[(row0 + r) + (int64_t)l * lda];
+    }
+    __syncthreads();
+    const int r = t % CH_PR, cg = t / CH_PR;  // 8 groups of 8 columns
+    if (r < rows) {
+#pragma unroll
+        for (int jj = 0; jj < CH_NB / 8; ++jj) {
+            const int j = cg * (CH_NB / 8) + jj;
+            if (j < kb) {
+                double acc = 0.0;
+                for (int l = 0; l <= j; ++l) acc += pr[r][l] * w[j][l];
+                P[(row0 + r) + (int64_t)j * lda] = acc;
+            }
+        }
+    }
+}
+
 __global__ void k_identity(double *__restrict__ X, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * n) return;
@@ -293,6 +401,7 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
     yy.alloc(ntot);
     xx.alloc(ntot);
     info.alloc(P);
+    chW.alloc(CH_NB * CH_NB);
     if (!h) rb(rocblas_create_handle(&h), "rocblas_create_handle");
     TG_HIP(hipStreamSynchronize(st), "tg_solver:coarse");
 }
@@ -312,7 +421,7 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
     // As_p (n_p x n_{p-1}, column-major) holds A_{p,p-1}.  Chain A turns As_{p+1} into
     // L_{p+1,p} = A_{p+1,p} L_pp^-T (right trsm); chain B turns As_p into X_p = L_pp^-1 A_{p,p-1}
     // = L_{p-1,p}^T (left trsm).
-    // Plane factor (rocSOLVER potrf) and its inverse inv(L) = L \ I by one triangular solve.  Every
+    // Plane factor (blocked Cholesky, kernels above) and its inverse inv(L) = L \ I by one triangular solve.  Every
     // call runs on the solver stream: with the two chains on two streams (two rocBLAS handles) the
     // factors came out different from run to run on ROCm 7.2 (wrong in the low to leading digits on
     // about one assembly in eight), however the streams were ordered; on one stream they are
@@ -320,7 +429,18 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
     auto factor_plane = [&](int p) {
         const rocblas_int np = (rocblas_int)nd[p];
         double *S = Ad.p + adOff[p];
-        rb(rocsolver_dpotrf(h, rocblas_fill_lower, np, S, np, info.p + p), "rocsolver_dpotrf");
+        for (int k0 = 0; k0 < np; k0 += CH_NB) {
+            const int kb = std::min<int>(CH_NB, np - k0), mr = np - k0 - kb;
+            double *D = S + k0 + (int64_t)k0 * np;
+            k_chol_diag<<<1, 256, 0, st>>>(D, np, kb, chW.p, info.p + p, k0);
+            if (mr > 0) k_chol_panel<<<(mr + CH_PR - 1) / CH_PR, 256, 0, st>>>(D + kb, np, mr, kb, chW.p);
+            TG_LAUNCH_CHECK(id);
+            if (mr > 0) {
+                rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, mr, kb, &mone, D + kb, np, &one,
+                                 D + kb + (int64_t)kb * np, np),
+                   "rocblas_dsyrk");
+            }
+        }
         k_identity<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Linv.p + adOff[p], np);
         TG_LAUNCH_CHECK(id);
         rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,

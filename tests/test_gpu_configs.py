@@ -125,9 +125,15 @@ def test_mixed_precision_cantilever_96x192x96_against_fp64():
 def test_value_noise_pcg_with_smaller_jacobi_weight_matches_oracle(tmp_path, non_dyadic):
     """The V-cycle smoother weight as a parameter (the reference's global weightFactorJacobi_): on the
     64 x 32 x 32 value-noise box, omega = 0.4 instead of 0.6 (0.6 exceeds 2 / lambda_max(D^-1 K)
-    there) cuts the first solve from 105 to 24 PCG iterations (54 to 18 on the dyadic hierarchy); the
-    device follows the oracle with the same weight (same count, compliance within 20x the oracle's
-    unit-roundoff spread)."""
+    there) cuts the first solve from 105 to 24 PCG iterations (54 to 18 on the dyadic hierarchy).
+
+    The device follows the oracle with the same weight: the same PCG count, or one fewer or more where
+    the oracle's relative residual at the earlier of the two stops lies within 5% of tol.  The
+    non-dyadic solve is such a near-tie: the oracle's residual after 23 iterations is 1.005e-3 on one
+    CPU and 1.020e-3 on another (its own rounding moves it by 1.5%), and rounding-level differences in
+    the coarsest factorisation decide it either way.  Run for the same number of iterations, the
+    compliance must agree within 20x the oracle's own spread under a unit-roundoff perturbation of the
+    load."""
     import torch
     from oracle import solver as S
     from top_gpu_on_matlab_amd.solver import Solver
@@ -142,11 +148,7 @@ def test_value_noise_pcg_with_smaller_jacobi_weight_matches_oracle(tmp_path, non
     osv = S.Solver(om, weight_factor_jacobi=0.4)
     osv.assemble(np.ones(om.levels[0].numElements))
     b = om.F[:, 0]
-    y_ref, its_ref, _ = osv.pcg(b, 1e-3, 800)
-    bp = b * (1 + 2.0 ** -52 * np.random.default_rng(11).standard_normal(b.shape))
-    y_per, its_per, _ = osv.pcg(bp, 1e-3, 800)
-    c_ref = float(b @ y_ref)
-    spread = abs(float(b @ y_per) - c_ref) / c_ref
+    _, its_ref, hist = osv.pcg(b, 1e-3, 800)
     gm = load_topvoxel_model(path, non_dyadic=non_dyadic)
     gsv = Solver(gm)
     gsv.set_modulus(torch.ones(gm.num_elements, dtype=torch.float64, device='cuda'))
@@ -154,10 +156,18 @@ def test_value_noise_pcg_with_smaller_jacobi_weight_matches_oracle(tmp_path, non
     its06 = gsv.pcg(b)[1]
     gsv.set_jacobi_weight(0.4)
     y, its, _ = gsv.pcg(b)
+    near_tie = abs(hist[min(its, its_ref) - 1] / 1e-3 - 1) < 0.05
+    # the oracle run for exactly `its` iterations (tol 0), and its unit-roundoff spread there
+    bp = b * (1 + 2.0 ** -52 * np.random.default_rng(11).standard_normal(b.shape))
+    y_same = osv.pcg(b, 0.0, its)[0]
+    y_per = osv.pcg(bp, 0.0, its)[0]
+    c_ref = float(b @ y_same)
+    spread = abs(float(b @ y_per) - c_ref) / c_ref
     dev = abs(float(b @ y.cpu().numpy()) - c_ref) / c_ref
-    print('\n[value-noise non_dyadic=%d] PCG its: omega 0.6 -> %d, omega 0.4 -> %d (oracle %d, perturbed %d); '
-          'compliance %.1e (oracle spread %.1e)' % (non_dyadic, its06, its, its_ref, its_per, dev, spread), flush=True)
-    assert abs(its - its_ref) <= 3 * abs(its_per - its_ref)
+    print('\n[value-noise non_dyadic=%d] PCG its: omega 0.6 -> %d, omega 0.4 -> %d (oracle %d, residual tail %s); '
+          'compliance after %d iterations %.1e (oracle spread %.1e)'
+          % (non_dyadic, its06, its, its_ref, ['%.4e' % v for v in hist[-2:]], its, dev, spread), flush=True)
+    assert its == its_ref or (abs(its - its_ref) == 1 and near_tie)
     assert dev < max(20 * spread, 1e-12)
     assert 2 * its < its06
 

@@ -59,71 +59,126 @@ constexpr int CH_PR = 32;  // panel rows per workgroup
 
 // info: set to col0 + j + 1 (LAPACK convention, first failure only) when pivot j is not positive;
 // the factorisation then continues with a unit pivot so every value stays finite.
+// Steps of CH_RB columns: every lane factorises the CH_RB x CH_RB diagonal sub-block itself (from LDS,
+// in registers), the rows below are scaled (one lane per row), then a rank-CH_RB trailing update;
+// the inverse is formed in row blocks of CH_IB with one lane per (row in block, column).  Two barriers
+// per step and one per inverse row block.
+constexpr int CH_RB = 4;  // columns per factorisation step
+constexpr int CH_IB = 4;  // rows per inverse block (CH_IB lanes per column: 4 x 64 = 256)
 __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64_t lda, int kb,
                                                    double *__restrict__ W, int *__restrict__ info, int col0) {
-    __shared__ double a[CH_NB][CH_NB + 1];  // a[i][j] = A(i, j); the diagonal is kept in dg
+    __shared__ double a[CH_NB][CH_NB + 1];  // a[i][j] = A(i, j), factorised in place
     __shared__ double w[CH_NB][CH_NB + 1];  // inv(L_kk)
-    __shared__ double dg[CH_NB];
     const int t = threadIdx.x;
     for (int idx = t; idx < kb * kb; idx += 256) {
         const int i = idx % kb, j = idx / kb;
         a[i][j] = A[i + (int64_t)j * lda];
     }
+    // rows/columns kb..CH_NB-1 of the padded block: identity, so every step can work on full blocks
+    for (int idx = t; idx < CH_NB * CH_NB; idx += 256) {
+        const int i = idx % CH_NB, j = idx / CH_NB;
+        if (i >= kb || j >= kb) a[i][j] = i == j ? 1.0 : 0.0;
+    }
     __syncthreads();
     int bad = 0;
-    const int ri = t % CH_NB, cph = t / CH_NB;  // trailing update: row ri, columns cph, cph + 4, ...
-    for (int j = 0; j < kb; ++j) {
-        double d = a[j][j];  // final since the previous step's update
-        if (!(d > 0.0)) {
-            if (!bad) bad = j + 1;
-            d = 1.0;
-        }
-        const double djj = sqrt(d);
-        if (t == 0) dg[j] = djj;
-        if (t > j && t < kb) a[t][j] /= djj;  // kb <= 64 < 256
-        __syncthreads();
-        // a(i, l) -= a(i, j) a(l, j) for j < l <= i < kb
-        if (ri > j && ri < kb) {
-            const double aij = a[ri][j];
-            int l = j + 1 + cph;
-            for (; l + 12 <= ri; l += 16) {  // four independent updates per trip: loads first
-                const double c0 = a[l][j], c1 = a[l + 4][j], c2 = a[l + 8][j], c3 = a[l + 12][j];
-                const double v0 = a[ri][l], v1 = a[ri][l + 4], v2 = a[ri][l + 8], v3 = a[ri][l + 12];
-                a[ri][l] = v0 - aij * c0;
-                a[ri][l + 4] = v1 - aij * c1;
-                a[ri][l + 8] = v2 - aij * c2;
-                a[ri][l + 12] = v3 - aij * c3;
+    for (int j0 = 0; j0 < CH_NB; j0 += CH_RB) {
+        // the diagonal sub-block, factorised redundantly by every lane
+        double L[CH_RB][CH_RB];
+#pragma unroll
+        for (int r = 0; r < CH_RB; ++r)
+#pragma unroll
+            for (int c = 0; c <= r; ++c) L[r][c] = a[j0 + r][j0 + c];
+#pragma unroll
+        for (int c = 0; c < CH_RB; ++c) {
+            double d = L[c][c];
+#pragma unroll
+            for (int k = 0; k < c; ++k) d -= L[c][k] * L[c][k];
+            if (!(d > 0.0)) {
+                if (!bad) bad = j0 + c + 1;
+                d = 1.0;
             }
-            for (; l <= ri; l += 4) a[ri][l] -= aij * a[l][j];
+            L[c][c] = sqrt(d);
+#pragma unroll
+            for (int r = c + 1; r < CH_RB; ++r) {
+                double v = L[r][c];
+#pragma unroll
+                for (int k = 0; k < c; ++k) v -= L[r][k] * L[c][k];
+                L[r][c] = v / L[c][c];
+            }
+        }
+        // rows below: x L^T = a(i, j0 .. j0+3)
+        const int i = j0 + CH_RB + t;
+        if (i < CH_NB) {
+            double x[CH_RB];
+#pragma unroll
+            for (int c = 0; c < CH_RB; ++c) {
+                double v = a[i][j0 + c];
+#pragma unroll
+                for (int k = 0; k < c; ++k) v -= x[k] * L[c][k];
+                x[c] = v / L[c][c];
+            }
+#pragma unroll
+            for (int c = 0; c < CH_RB; ++c) a[i][j0 + c] = x[c];
+        }
+        __syncthreads();  // every lane has read the sub-block; the scaled columns are complete
+        if (t == 0) {
+#pragma unroll
+            for (int r = 0; r < CH_RB; ++r)
+#pragma unroll
+                for (int c = 0; c <= r; ++c) a[j0 + r][j0 + c] = L[r][c];
+        }
+        // trailing update of the lower triangle past the step: lane (row ri, column phase cph)
+        const int ri = t % CH_NB, cph = t / CH_NB;
+        if (ri >= j0 + CH_RB) {
+            double ar[CH_RB];
+#pragma unroll
+            for (int c = 0; c < CH_RB; ++c) ar[c] = a[ri][j0 + c];
+            for (int l = j0 + CH_RB + cph; l <= ri; l += 4) {
+                double v = a[ri][l];
+#pragma unroll
+                for (int c = 0; c < CH_RB; ++c) v -= ar[c] * a[l][j0 + c];
+                a[ri][l] = v;
+            }
         }
         __syncthreads();
     }
-    // W = inv(L_kk) row by row; four lanes per column split each dot product
-    const int wc = t / 4, part = t % 4;
-    for (int i = 0; i < kb; ++i) {
-        double acc = 0.0, acc2 = 0.0;
-        if (wc < i) {
-            int k = wc + part;
-            for (; k + 4 < i; k += 8) {  // two chains, loads first
-                const double p0 = a[i][k], q0 = w[k][wc], p1 = a[i][k + 4], q1 = w[k + 4][wc];
-                acc += p0 * q0;
-                acc2 += p1 * q1;
-            }
-            if (k < i) acc += a[i][k] * w[k][wc];
+    // W = inv(L): row block i0 .. i0+3, lane (r = t % 4, column c = t / 4); the four lanes of a column
+    // are adjacent and pass the block's new rows along by shuffles
+    const int r = t % CH_IB, c = t / CH_IB;
+    for (int i0 = 0; i0 < CH_NB; i0 += CH_IB) {
+        const int i = i0 + r;
+        double sacc = 0.0, sacc2 = 0.0;  // sum over the finished rows k < i0 of L(i, k) W(k, c)
+        int k = c;
+        for (; k + 1 < i0; k += 2) {
+            sacc += a[i][k] * w[k][c];
+            sacc2 += a[i][k + 1] * w[k + 1][c];
         }
-        acc += acc2;
-        acc += __shfl_xor(acc, 1, 4);
-        acc += __shfl_xor(acc, 2, 4);
-        if (part == 0 && wc < kb) w[i][wc] = wc == i ? 1.0 / dg[i] : (wc < i ? -acc / dg[i] : 0.0);
+        if (k < i0) sacc += a[i][k] * w[k][c];
+        sacc += sacc2;
+        double wv = 0.0;
+#pragma unroll
+        for (int rr = 0; rr < CH_IB; ++rr) {
+            // row i0 + rr: lane rr finishes it with the block rows above it (held by lanes < rr)
+            double add = 0.0;
+#pragma unroll
+            for (int q = 0; q < rr; ++q) {
+                const double wq = __shfl(wv, (t & ~(CH_IB - 1)) + q, 64);
+                if (r == rr) add += a[i][i0 + q] * wq;
+            }
+            if (r == rr) {
+                const double lii = a[i][i];
+                wv = c > i ? 0.0 : (c == i ? 1.0 / lii : -(sacc + add) / lii);
+            }
+        }
+        w[i][c] = wv;
         __syncthreads();
     }
     for (int idx = t; idx < kb * kb; idx += 256) {
-        const int i = idx % kb, j = idx / kb;
-        if (i > j) A[i + (int64_t)j * lda] = a[i][j];
-        else if (i == j) A[i + (int64_t)j * lda] = dg[i];
-        W[i + CH_NB * j] = w[i][j];
+        const int ii = idx % kb, j = idx / kb;
+        if (ii >= j) A[ii + (int64_t)j * lda] = a[ii][j];
+        W[ii + CH_NB * j] = w[ii][j];
     }
-    if (t == 0 && bad && *info == 0) *info = col0 + bad;
+    if (t == 0 && bad && bad <= kb && *info == 0) *info = col0 + bad;
 }
 
 // Rows under the diagonal block: X = P W^T for the m x kb panel P (column-major, lda), in place;

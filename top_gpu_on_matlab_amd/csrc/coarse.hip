@@ -65,11 +65,17 @@ constexpr int CH_PR = 32;  // panel rows per workgroup
 // per step and one per inverse row block.
 constexpr int CH_RB = 4;  // columns per factorisation step
 constexpr int CH_IB = 4;  // rows per inverse block (CH_IB lanes per column: 4 x 64 = 256)
+// Batched over blockIdx.y: problem y works on A + y * dA, W + y * CH_NB^2 and info[y * dinfo] (the two
+// elimination chains' planes of one step share a launch).
 __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64_t lda, int kb,
-                                                   double *__restrict__ W, int *__restrict__ info, int col0) {
+                                                   double *__restrict__ W, int *__restrict__ info, int col0,
+                                                   int64_t dA, int dinfo) {
     __shared__ double a[CH_NB][CH_NB + 1];  // a[i][j] = A(i, j), factorised in place
     __shared__ double w[CH_NB][CH_NB + 1];  // inv(L_kk)
     const int t = threadIdx.x;
+    A += blockIdx.y * dA;
+    W += blockIdx.y * (CH_NB * CH_NB);
+    info += blockIdx.y * dinfo;
     for (int idx = t; idx < kb * kb; idx += 256) {
         const int i = idx % kb, j = idx / kb;
         a[i][j] = A[i + (int64_t)j * lda];
@@ -184,10 +190,12 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64
 // Rows under the diagonal block: X = P W^T for the m x kb panel P (column-major, lda), in place;
 // CH_PR rows per workgroup, lane (row r, column group cg) forms 8 outputs.
 __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ P, int64_t lda, int m, int kb,
-                                                    const double *__restrict__ W) {
+                                                    const double *__restrict__ W, int64_t dA) {
     __shared__ double w[CH_NB][CH_NB + 1];
     __shared__ double pr[CH_PR][CH_NB + 1];
     const int t = threadIdx.x;
+    P += blockIdx.y * dA;  // batched like k_chol_diag
+    W += blockIdx.y * (CH_NB * CH_NB);
     const int row0 = blockIdx.x * CH_PR;
     const int rows = min(CH_PR, m - row0);
     for (int idx = t; idx < kb * kb; idx += 256) {
@@ -456,7 +464,7 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
     yy.alloc(ntot);
     xx.alloc(ntot);
     info.alloc(P);
-    chW.alloc(CH_NB * CH_NB);
+    chW.alloc(2 * CH_NB * CH_NB);
     if (!h) rb(rocblas_create_handle(&h), "rocblas_create_handle");
     TG_HIP(hipStreamSynchronize(st), "tg_solver:coarse");
 }
@@ -481,49 +489,64 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
     // factors came out different from run to run on ROCm 7.2 (wrong in the low to leading digits on
     // about one assembly in eight), however the streams were ordered; on one stream they are
     // bitwise reproducible.
-    auto factor_plane = [&](int p) {
+    // Factor plane p, or planes p and q (same size) together: the LDS kernels take both in one launch
+    // (blockIdx.y); the rocBLAS calls stay per plane.
+    auto factor_planes = [&](int p, int q) {
         const rocblas_int np = (rocblas_int)nd[p];
-        double *S = Ad.p + adOff[p];
+        const int nb = q >= 0 ? 2 : 1;
+        const int64_t dA = q >= 0 ? adOff[q] - adOff[p] : 0;
+        const int dinfo = q >= 0 ? q - p : 0;
         for (int k0 = 0; k0 < np; k0 += CH_NB) {
             const int kb = std::min<int>(CH_NB, np - k0), mr = np - k0 - kb;
-            double *D = S + k0 + (int64_t)k0 * np;
-            k_chol_diag<<<1, 256, 0, st>>>(D, np, kb, chW.p, info.p + p, k0);
-            if (mr > 0) k_chol_panel<<<(mr + CH_PR - 1) / CH_PR, 256, 0, st>>>(D + kb, np, mr, kb, chW.p);
+            double *D = Ad.p + adOff[p] + k0 + (int64_t)k0 * np;
+            k_chol_diag<<<dim3(1, nb), 256, 0, st>>>(D, np, kb, chW.p, info.p + p, k0, dA, dinfo);
+            if (mr > 0)
+                k_chol_panel<<<dim3((mr + CH_PR - 1) / CH_PR, nb), 256, 0, st>>>(D + kb, np, mr, kb, chW.p, dA);
             TG_LAUNCH_CHECK(id);
-            if (mr > 0) {
-                rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, mr, kb, &mone, D + kb, np, &one,
-                                 D + kb + (int64_t)kb * np, np),
+            for (int y = 0; y < nb && mr > 0; ++y)
+                rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, mr, kb, &mone, D + y * dA + kb, np,
+                                 &one, D + y * dA + kb + (int64_t)kb * np, np),
                    "rocblas_dsyrk");
-            }
         }
-        k_identity<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Linv.p + adOff[p], np);
-        TG_LAUNCH_CHECK(id);
-        rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, rocblas_diagonal_non_unit,
-                         np, np, &one, S, np, Linv.p + adOff[p], np), "rocblas_dtrsm inverse");
-    };
-    for (int p = 0; p < m; ++p) {  // chain A
-        const rocblas_int np = (rocblas_int)nd[p];
-        if (np == 0) continue;
-        if (p > 0 && nd[p - 1] > 0)
-            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, np, (rocblas_int)nd[p - 1], &mone,
-                             As.p + asOff[p], np, &one, Ad.p + adOff[p], np), "rocblas_dsyrk");
-        factor_plane(p);
-        if (nd[p + 1] > 0)
-            rb(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                             rocblas_diagonal_non_unit, (rocblas_int)nd[p + 1], np, &one, Ad.p + adOff[p], np,
-                             As.p + asOff[p + 1], (rocblas_int)nd[p + 1]), "rocblas_dtrsm A");
-    }
-    for (int p = P - 1; p > m; --p) {  // chain B
-        const rocblas_int np = (rocblas_int)nd[p];
-        if (np == 0) continue;
-        if (p + 1 < P && nd[p + 1] > 0)  // S_p -= X_{p+1}^T X_{p+1}
-            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, np, (rocblas_int)nd[p + 1], &mone,
-                             As.p + asOff[p + 1], (rocblas_int)nd[p + 1], &one, Ad.p + adOff[p], np), "rocblas_dsyrk");
-        factor_plane(p);
-        if (nd[p - 1] > 0)
+        for (int y = 0; y < nb; ++y) {
+            const int pp = y ? q : p;
+            k_identity<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Linv.p + adOff[pp], np);
+            TG_LAUNCH_CHECK(id);
             rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                             rocblas_diagonal_non_unit, np, (rocblas_int)nd[p - 1], &one, Ad.p + adOff[p], np,
-                             As.p + asOff[p], np), "rocblas_dtrsm B");
+                             rocblas_diagonal_non_unit, np, np, &one, Ad.p + adOff[pp], np, Linv.p + adOff[pp], np),
+               "rocblas_dtrsm inverse");
+        }
+    };
+    // chains A (p = 0 .. m-1) and B (q = P-1 .. m+1) advance together, one plane each per step
+    const int lenA = m, lenB = P - 1 - m;
+    for (int k = 0; k < std::max(lenA, lenB); ++k) {
+        const int p = k < lenA && nd[k] > 0 ? k : -1;
+        const int q = k < lenB && nd[P - 1 - k] > 0 ? P - 1 - k : -1;
+        if (p > 0 && nd[p - 1] > 0)  // S_p -= L_{p,p-1} L_{p,p-1}^T
+            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)nd[p], (rocblas_int)nd[p - 1],
+                             &mone, As.p + asOff[p], (rocblas_int)nd[p], &one, Ad.p + adOff[p], (rocblas_int)nd[p]),
+               "rocblas_dsyrk");
+        if (q >= 0 && q + 1 < P && nd[q + 1] > 0)  // S_q -= X_{q+1}^T X_{q+1}
+            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, (rocblas_int)nd[q],
+                             (rocblas_int)nd[q + 1], &mone, As.p + asOff[q + 1], (rocblas_int)nd[q + 1], &one,
+                             Ad.p + adOff[q], (rocblas_int)nd[q]),
+               "rocblas_dsyrk");
+        if (p >= 0 && q >= 0 && nd[p] == nd[q]) {
+            factor_planes(p, q);
+        } else {
+            if (p >= 0) factor_planes(p, -1);
+            if (q >= 0) factor_planes(q, -1);
+        }
+        if (p >= 0 && nd[p + 1] > 0)  // L_{p+1,p} = A_{p+1,p} L_pp^-T
+            rb(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                             rocblas_diagonal_non_unit, (rocblas_int)nd[p + 1], (rocblas_int)nd[p], &one,
+                             Ad.p + adOff[p], (rocblas_int)nd[p], As.p + asOff[p + 1], (rocblas_int)nd[p + 1]),
+               "rocblas_dtrsm A");
+        if (q >= 0 && nd[q - 1] > 0)  // X_q = L_qq^-1 A_{q,q-1}
+            rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                             rocblas_diagonal_non_unit, (rocblas_int)nd[q], (rocblas_int)nd[q - 1], &one,
+                             Ad.p + adOff[q], (rocblas_int)nd[q], As.p + asOff[q], (rocblas_int)nd[q]),
+               "rocblas_dtrsm B");
     }
     if (nd[m] > 0) {  // separator
         const rocblas_int nm = (rocblas_int)nd[m];
@@ -533,7 +556,7 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
         if (m + 1 < P && nd[m + 1] > 0)
             rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, nm, (rocblas_int)nd[m + 1], &mone,
                              As.p + asOff[m + 1], (rocblas_int)nd[m + 1], &one, Ad.p + adOff[m], nm), "rocblas_dsyrk");
-        factor_plane(m);
+        factor_planes(m, -1);
     }
     // Solve operators, row-major with zero-padded rows (column-major results of transposed products):
     //   LT_p = inv(L_pp) (row-major copy); inv(L_pp)^T is the column-major Linv itself.

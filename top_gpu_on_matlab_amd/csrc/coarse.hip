@@ -221,6 +221,54 @@ __global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ P, int6
     }
 }
 
+// Trailing update C -= P P^T of the lower triangle (C m x m, P m x kb, both column-major with lda),
+// one 64 x 64 tile of C per workgroup (tiles on and below the diagonal, column by column), each lane
+// 4 x 4 entries; batched over blockIdx.y like k_chol_diag (problem y at P + y dA, C + y dA).
+__global__ __launch_bounds__(256) void k_chol_update(const double *__restrict__ P, double *__restrict__ C, int64_t lda,
+                                                     int m, int kb, int64_t dA) {
+    __shared__ double pi[CH_NB][CH_NB + 1];  // pi[r][k] = P(I0 + r, k)
+    __shared__ double pj[CH_NB][CH_NB + 1];  // pj[r][k] = P(J0 + r, k)
+    P += blockIdx.y * dA;
+    C += blockIdx.y * dA;
+    const int nT = (m + CH_NB - 1) / CH_NB;
+    int tile = blockIdx.x, J = 0;
+    while (tile >= nT - J) {
+        tile -= nT - J;
+        ++J;
+    }
+    const int I = J + tile;
+    const int I0 = I * CH_NB, J0 = J * CH_NB;
+    const int t = threadIdx.x;
+    for (int idx = t; idx < CH_NB * kb; idx += 256) {
+        const int r = idx % CH_NB, k = idx / CH_NB;
+        pi[r][k] = I0 + r < m ? P[(I0 + r) + (int64_t)k * lda] : 0.0;
+        pj[r][k] = J0 + r < m ? P[(J0 + r) + (int64_t)k * lda] : 0.0;
+    }
+    __syncthreads();
+    const int ty = t % 16, tx = t / 16;  // rows I0 + 4 ty + i, columns J0 + 4 tx + j
+    double acc[4][4] = {};
+    for (int k = 0; k < kb; ++k) {
+        double a[4], b[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[i] = pi[4 * ty + i][k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) b[j] = pj[4 * tx + j][k];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = J0 + 4 * tx + j;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = I0 + 4 * ty + i;
+            if (row < m && col < m && row >= col) C[row + (int64_t)col * lda] -= acc[i][j];
+        }
+    }
+}
+
 __global__ void k_identity(double *__restrict__ X, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * n) return;
@@ -503,10 +551,12 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
             if (mr > 0)
                 k_chol_panel<<<dim3((mr + CH_PR - 1) / CH_PR, nb), 256, 0, st>>>(D + kb, np, mr, kb, chW.p, dA);
             TG_LAUNCH_CHECK(id);
-            for (int y = 0; y < nb && mr > 0; ++y)
-                rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, mr, kb, &mone, D + y * dA + kb, np,
-                                 &one, D + y * dA + kb + (int64_t)kb * np, np),
-                   "rocblas_dsyrk");
+            if (mr > 0) {
+                const int nT = (mr + CH_NB - 1) / CH_NB;
+                k_chol_update<<<dim3(nT * (nT + 1) / 2, nb), 256, 0, st>>>(D + kb, D + kb + (int64_t)kb * np, np, mr,
+                                                                        kb, dA);
+                TG_LAUNCH_CHECK(id);
+            }
         }
         for (int y = 0; y < nb; ++y) {
             const int pp = y ? q : p;

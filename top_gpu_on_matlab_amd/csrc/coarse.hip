@@ -72,6 +72,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64
                                                    int64_t dA, int dinfo) {
     __shared__ double a[CH_NB][CH_NB + 1];  // a[i][j] = A(i, j), factorised in place
     __shared__ double w[CH_NB][CH_NB + 1];  // inv(L_kk)
+    __shared__ double rdg[CH_NB];           // 1 / L(j, j)
     const int t = threadIdx.x;
     A += blockIdx.y * dA;
     W += blockIdx.y * (CH_NB * CH_NB);
@@ -89,7 +90,8 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64
     int bad = 0;
     for (int j0 = 0; j0 < CH_NB; j0 += CH_RB) {
         // the diagonal sub-block, factorised redundantly by every lane
-        double L[CH_RB][CH_RB];
+        // (one division per column; the rest of the step multiplies by the reciprocal pivot)
+        double L[CH_RB][CH_RB], rp[CH_RB];
 #pragma unroll
         for (int r = 0; r < CH_RB; ++r)
 #pragma unroll
@@ -104,12 +106,13 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64
                 d = 1.0;
             }
             L[c][c] = sqrt(d);
+            rp[c] = 1.0 / L[c][c];
 #pragma unroll
             for (int r = c + 1; r < CH_RB; ++r) {
                 double v = L[r][c];
 #pragma unroll
                 for (int k = 0; k < c; ++k) v -= L[r][k] * L[c][k];
-                L[r][c] = v / L[c][c];
+                L[r][c] = v * rp[c];
             }
         }
         // rows below: x L^T = a(i, j0 .. j0+3)
@@ -121,7 +124,7 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64
                 double v = a[i][j0 + c];
 #pragma unroll
                 for (int k = 0; k < c; ++k) v -= x[k] * L[c][k];
-                x[c] = v / L[c][c];
+                x[c] = v * rp[c];
             }
 #pragma unroll
             for (int c = 0; c < CH_RB; ++c) a[i][j0 + c] = x[c];
@@ -129,9 +132,11 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64
         __syncthreads();  // every lane has read the sub-block; the scaled columns are complete
         if (t == 0) {
 #pragma unroll
-            for (int r = 0; r < CH_RB; ++r)
+            for (int r = 0; r < CH_RB; ++r) {
 #pragma unroll
                 for (int c = 0; c <= r; ++c) a[j0 + r][j0 + c] = L[r][c];
+                rdg[j0 + r] = rp[r];
+            }
         }
         // trailing update of the lower triangle past the step: lane (row ri, column phase cph)
         const int ri = t % CH_NB, cph = t / CH_NB;
@@ -172,8 +177,8 @@ __global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64
                 if (r == rr) add += a[i][i0 + q] * wq;
             }
             if (r == rr) {
-                const double lii = a[i][i];
-                wv = c > i ? 0.0 : (c == i ? 1.0 / lii : -(sacc + add) / lii);
+                const double rii = rdg[i];
+                wv = c > i ? 0.0 : (c == i ? rii : -(sacc + add) * rii);
             }
         }
         w[i][c] = wv;

@@ -38,7 +38,7 @@ enum {
     TG_ERR_ARGS = 1,   /* wrong number of inputs/outputs, null pointer, bad option */
     TG_ERR_TYPE = 2,   /* class mismatch (single/double/int32/uint32) */
     TG_ERR_SHAPE = 3,  /* dimension mismatch */
-    TG_ERR_HIP = 4,    /* HIP runtime / rocBLAS / rocSOLVER failure */
+    TG_ERR_HIP = 4,    /* HIP runtime / rocBLAS failure             */
     TG_ERR_STATE = 5,  /* call out of order (e.g. solve before assemble) */
     TG_ERR_NUMERIC = 6 /* coarsest matrix not positive definite, non-finite values */
 };

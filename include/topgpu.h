@@ -38,7 +38,7 @@ enum {
     TG_ERR_ARGS = 1,   /* wrong number of inputs/outputs, null pointer, bad option */
     TG_ERR_TYPE = 2,   /* class mismatch (single/double/int32/uint32) */
     TG_ERR_SHAPE = 3,  /* dimension mismatch */
-    TG_ERR_HIP = 4,    /* HIP runtime / rocBLAS failure             */
+    TG_ERR_HIP = 4,    /* HIP runtime / rocBLAS failure */
     TG_ERR_STATE = 5,  /* call out of order (e.g. solve before assemble) */
     TG_ERR_NUMERIC = 6 /* coarsest matrix not positive definite, non-finite values */
 };
@@ -220,8 +220,11 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *   TG_TUNE_TIME_KFINE    1 = bracket every tiled finest K*u launch with HIP events, 2 = only the
  *                         plain products (the PCG operator), 0 = stop; 1 and 2 clear the record
  *   TG_TUNE_KELEM_PARTS   threads per element of the per-element-matrix K*u (coarse levels and the
- *                         super-element finest level): 4, 6, 8, 12 or 24 (default 24) */
-enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4 };
+ *                         super-element finest level): 4, 6, 8, 12 or 24 (default 24)
+ *   TG_TUNE_COARSE_GRAPH  1 = replay the coarsest solve's launch sequence as a captured HIP graph,
+ *                         0 = launch it kernel by kernel (default: measured 1% faster per PCG iteration) */
+enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4,
+       TG_TUNE_COARSE_GRAPH = 5 };
 int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value);
 /* Damped-Jacobi weight of the V-cycle smoother (the reference's global weightFactorJacobi_,
  * TOP_GPU.m:397, 784, 809-811; default 0.6).  0 < omega < 2.  The smoother only reduces every error

@@ -136,6 +136,22 @@ def test_coarse_solve(pair):
     assert _rel(gsv.coarse_solve(r).cpu().numpy(), osv.coarse_solve(r)) < 1e-9
 
 
+def test_coarse_solve_graph_replay_matches_direct_launches(pair):
+    """The coarsest solve launched kernel by kernel and replayed as a captured HIP graph run the same
+    kernels on the same buffers: bitwise-identical results, on repeated replays."""
+    name, om, osv, gm, gsv, E = pair
+    r = np.random.default_rng(4).standard_normal(3 * om.levels[-1].numNodes)
+    direct = gsv.coarse_solve(r).clone()
+    try:
+        gsv.set_tuning(coarse_graph=True)
+        for k in range(3):
+            assert gsv.coarse_solve(r).equal(direct), (name, k)
+            assert gsv.coarse_solve(2.0 * r).equal(2.0 * direct), (name, k)   # scaling by 2 is exact
+    finally:
+        gsv.set_tuning(coarse_graph=False)
+    assert gsv.coarse_solve(r).equal(direct)
+
+
 def test_vcycle(pair):
     name, om, osv, gm, gsv, E = pair
     r = np.random.default_rng(4).standard_normal(3 * om.levels[0].numNodes)

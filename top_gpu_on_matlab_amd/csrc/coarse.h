@@ -6,12 +6,12 @@
 //   chain A, p = 0 .. m-1:     L_pp L_pp^T = A_pp - L_{p,p-1} L_{p,p-1}^T,  L_{p+1,p} = A_{p+1,p} L_pp^-T
 //   chain B, p = P-1 .. m+1:   L_pp L_pp^T = A_pp - L_{p,p+1} L_{p,p+1}^T,  L_{p-1,p} = A_{p-1,p} L_pp^-T
 //   separator:                 L_mm L_mm^T = A_mm - L_{m,m-1} L_{m,m-1}^T - L_{m,m+1} L_{m,m+1}^T
-// on the device (blocked Cholesky: LDS kernels for the diagonal blocks and panels, rocBLAS syrk for
-// the trailing updates; rocBLAS trsm/gemm for the rest; all on the solver stream).  A solve is a batched triangular
-// product y = blockdiag(inv(L_pp)) b, the two forward chains advanced together (one launch per step,
-// ~P/2 steps) and the separator, then x = blockdiag(inv(L_pp)^T) y and the two backward chains
-// outwards from the separator; the whole sequence is captured once as a HIP graph (its buffers are
-// fixed at setup) and replayed on every solve.
+// on the device (blocked Cholesky: LDS kernels for the diagonal blocks, panels and trailing updates,
+// the two chains' planes of a step sharing each launch; rocBLAS syrk/trsm/gemm for the couplings; all on
+// the solver stream).  A solve is a batched triangular product y = blockdiag(inv(L_pp)) b, the two
+// forward chains advanced together (one launch per step, ~P/2 steps) and the separator, then
+// x = blockdiag(inv(L_pp)^T) y and the two backward chains outwards from the separator.  The sequence
+// reads only buffers fixed at setup, so it can also be captured once as a HIP graph and replayed.
 #pragma once
 
 #include <rocblas/rocblas.h>
@@ -34,7 +34,9 @@ struct CoarseSolver {
     Grid g{};
     DevBuf<int32_t> lidx;           // per full-grid node: local node index within its plane (-1 none)
     DevBuf<int64_t> d_off, d_adOff, d_asOff;
-    DevBuf<int32_t> rowPlane;       // plane of each row of the stacked vector
+    int64_t npairs = 0;             // row pairs (r, n_p - 1 - r) over all planes
+    DevBuf<int32_t> pairPlane;      // plane of each row pair
+    DevBuf<int64_t> d_pairOff;      // first row pair of each plane
     DevBuf<double> Ad, As, Linv, LT, Fop, Bop;
     DevBuf<double> bb, yy, xx;
     DevBuf<rocblas_int> info;
@@ -42,6 +44,7 @@ struct CoarseSolver {
     rocblas_handle h = nullptr;
     bool factored = false;
     hipGraphExec_t exec = nullptr;  // the captured solve sequence
+    bool use_graph = false;         // replay the captured sequence (false: launch it directly)
 
     ~CoarseSolver();
     void setup(const HostLevel &L, const Grid &grid, hipStream_t st);

@@ -52,8 +52,8 @@ __global__ void k_coarse_assemble(const double *__restrict__ Ks, int64_t ldk, co
 
 // ---- plane-block Cholesky: blocked right-looking, panels of CH_NB columns --------------------
 // Per panel: the kb x kb diagonal block is factorised in LDS by one workgroup, which also forms
-// W = inv(L_kk); the rows under it are X = P W^T (k_chol_panel); the trailing matrix is updated by a
-// rocBLAS syrk.  Column-major, lower triangle; the upper triangle is not referenced.
+// W = inv(L_kk); the rows under it are X = P W^T (k_chol_panel); the trailing matrix is updated by
+// k_chol_update.  Column-major, lower triangle; the upper triangle is not referenced.
 constexpr int CH_NB = 64;
 constexpr int CH_PR = 32;  // panel rows per workgroup
 
@@ -288,37 +288,56 @@ __global__ void k_transpose_block(const double *__restrict__ A, double *__restri
     Tr[i] = A[r + n * c];
 }
 
-// Batched triangular products over all planes, one wavefront per row (row R of the stacked
-// vector, plane rowPlane[R]):  out_p = Tri_p * in_p, Tri_p row-major n_p x n_p, lower (columns
-// 0..r) or upper (columns r..n_p-1) triangular.
-__global__ __launch_bounds__(256) void k_tri_rows(const double *__restrict__ Tri, const int64_t *__restrict__ adOff,
-                                                  const int64_t *__restrict__ off, const int32_t *__restrict__ rowPlane,
-                                                  int64_t ntot, int upper, const double *__restrict__ in,
-                                                  double *__restrict__ out) {
+// Batched triangular products over all planes, out_p = Tri_p * in_p with Tri_p row-major n_p x n_p,
+// lower (columns 0..r) or upper (columns r..n_p-1) triangular.  One wavefront per pair of rows
+// (r, n_p - 1 - r) of a plane (pair Q, plane pairPlane[Q], r = Q - pairOff[p]; the middle row of an
+// odd n_p alone): the pair holds n_p + 1 entries, so every wave streams the same amount.  The pair's
+// two row segments are walked as one run of n_p + 1 entries, 8 x 64 per round with every load issued
+// before the first use (clamped indices).
+__global__ __launch_bounds__(256) void k_tri_pairs(const double *__restrict__ Tri, const int64_t *__restrict__ adOff,
+                                                   const int64_t *__restrict__ off, const int32_t *__restrict__ pairPlane,
+                                                   const int64_t *__restrict__ pairOff, int64_t npairs, int upper,
+                                                   const double *__restrict__ in, double *__restrict__ out) {
     const int lane = threadIdx.x & 63;
-    const int64_t R = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (R >= ntot) return;
-    const int p = rowPlane[R];
-    const int64_t o = off[p], n = off[p + 1] - o, r = R - o;
-    const double *row = Tri + adOff[p] + r * n;
-    const double *x = in + o;
-    const int64_t c0 = upper ? r : 0, c1 = upper ? n : r + 1;
-    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t c = c0 + lane; c < c1; c += 8 * 64) {
-        // loads are unconditional (clamped index) so all eight are in flight before the first use
+    const int64_t Q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (Q >= npairs) return;
+    const int p = pairPlane[Q];
+    const int64_t o = off[p], n = off[p + 1] - o;
+    const int64_t r1 = Q - pairOff[p], r2 = n - 1 - r1;
+    const int64_t len1 = upper ? n - r1 : r1 + 1;
+    const int64_t len2 = r2 == r1 ? 0 : (upper ? n - r2 : r2 + 1);
+    const int64_t c1 = upper ? r1 : 0, c2 = upper ? r2 : 0;
+    const double *row1 = Tri + adOff[p] + r1 * n + c1, *x1 = in + o + c1;
+    const double *row2 = Tri + adOff[p] + r2 * n + c2 - len1, *x2 = in + o + c2 - len1;  // indexed by j >= len1
+    const int64_t tot = len1 + len2;
+    double s1[8] = {0, 0, 0, 0, 0, 0, 0, 0}, s2[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t j0 = lane; j0 < tot; j0 += 8 * 64) {
         double a[8], b[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            const int64_t cc = min(c + 64 * k, c1 - 1);
-            a[k] = row[cc];
-            b[k] = x[cc];
+            const int64_t j = min(j0 + 64 * k, tot - 1);
+            const bool first = j < len1;
+            a[k] = first ? row1[j] : row2[j];
+            b[k] = first ? x1[j] : x2[j];
         }
 #pragma unroll
-        for (int k = 0; k < 8; ++k) s[k] += (c + 64 * k < c1) ? a[k] * b[k] : 0.0;
+        for (int k = 0; k < 8; ++k) {
+            const int64_t j = j0 + 64 * k;
+            const double v = j < tot ? a[k] * b[k] : 0.0;
+            if (j < len1) s1[k] += v;
+            else s2[k] += v;
+        }
     }
-    double t = ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
-    for (int o2 = 32; o2 > 0; o2 >>= 1) t += __shfl_down(t, o2, 64);
-    if (lane == 0) out[R] = t;
+    double t1 = ((s1[0] + s1[1]) + (s1[2] + s1[3])) + ((s1[4] + s1[5]) + (s1[6] + s1[7]));
+    double t2 = ((s2[0] + s2[1]) + (s2[2] + s2[3])) + ((s2[4] + s2[5]) + (s2[6] + s2[7]));
+    for (int o2 = 32; o2 > 0; o2 >>= 1) {
+        t1 += __shfl_down(t1, o2, 64);
+        t2 += __shfl_down(t2, o2, 64);
+    }
+    if (lane == 0) {
+        out[o + r1] = t1;
+        if (len2) out[o + r2] = t2;
+    }
 }
 
 // One step of a block recurrence, one wavefront per row:  y[r] = c[r] - sum_j M[r * ld + j] v(j), with
@@ -497,14 +516,19 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
         fOff[p + 1] = fOff[p] + nd[p] * fLd[p];
         bOff[p + 1] = bOff[p] + nd[p] * bLd[p];
     }
-    std::vector<int32_t> rp(std::max<int64_t>(ntot, 1), 0);
+    // row pairs (r, n_p - 1 - r) of the batched triangular products
+    std::vector<int64_t> pOff(P + 1, 0);
+    for (int p = 0; p < P; ++p) pOff[p + 1] = pOff[p] + (nd[p] + 1) / 2;
+    npairs = pOff[P];
+    std::vector<int32_t> pp(std::max<int64_t>(npairs, 1), 0);
     for (int p = 0; p < P; ++p)
-        for (int64_t R = off[p]; R < off[p + 1]; ++R) rp[R] = p;
+        for (int64_t Q = pOff[p]; Q < pOff[p + 1]; ++Q) pp[Q] = p;
     lidx.upload(lid, st);
     d_off.upload(off, st);
     d_adOff.upload(adOff, st);
     d_asOff.upload(asOff, st);
-    rowPlane.upload(rp, st);
+    pairPlane.upload(pp, st);
+    d_pairOff.upload(pOff, st);
     Ad.alloc(adOff[P]);
     Linv.alloc(adOff[P]);
     LT.alloc(adOff[P]);
@@ -659,7 +683,7 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
 //   the separator; xx = blockdiag(Linv^T) yy; backward chains A (p = m-1 .. 0) and B (p = m+1 .. P-1).
 void CoarseSolver::enqueue_core(hipStream_t s) const {
     const char *id = "tg_solver:coarse";
-    const unsigned gr = grid_for(ntot, 4);
+    const unsigned gr = grid_for(npairs, 4);
     const ChainProb none = chain_prob(nullptr, 0, 0, 0, 0, nullptr, nullptr, nullptr, nullptr);
     auto fwd = [&](int p, int q) -> ChainProb {  // y_p -= Fop_p y_q
         if (p < 0 || p >= P || nd[p] == 0 || q < 0 || q >= P || nd[q] == 0) return none;
@@ -671,7 +695,7 @@ void CoarseSolver::enqueue_core(hipStream_t s) const {
         return chain_prob(Bop.p + bOff[p], nd[p], nd[q], nd[q], bLd[p], xx.p + off[p], xx.p + off[q], nullptr,
                           xx.p + off[p]);
     };
-    k_tri_rows<<<gr, 256, 0, s>>>(LT, d_adOff, d_off, rowPlane, ntot, 0, bb, yy);
+    k_tri_pairs<<<gr, 256, 0, s>>>(LT, d_adOff, d_off, pairPlane, d_pairOff, npairs, 0, bb, yy);
     const int lenA = m, lenB = P - 1 - m;  // planes per chain
     for (int k = 1; k < std::max(lenA, lenB); ++k)
         launch_chain(k < lenA ? fwd(k, k - 1) : none, k < lenB ? fwd(P - 1 - k, P - k) : none, s);
@@ -682,7 +706,7 @@ void CoarseSolver::enqueue_core(hipStream_t s) const {
                                     na ? yy.p + off[m - 1] : yy.p, nb ? yy.p + off[m + 1] : yy.p, yy.p + off[m]),
                          none, s);
     }
-    k_tri_rows<<<gr, 256, 0, s>>>(Linv, d_adOff, d_off, rowPlane, ntot, 1, yy, xx);
+    k_tri_pairs<<<gr, 256, 0, s>>>(Linv, d_adOff, d_off, pairPlane, d_pairOff, npairs, 1, yy, xx);
     for (int k = 1; k <= std::max(lenA, lenB); ++k)
         launch_chain(k <= lenA ? bwd(m - k, m - k + 1) : none, k <= lenB ? bwd(m + k, m + k - 1) : none, s);
     TG_LAUNCH_CHECK(id);
@@ -694,7 +718,10 @@ void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t 
     if (!factored) fail(TG_ERR_STATE, id, "coarsest level not factorised (call assemble first)");
     k_pack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(r, bb, nodeMask, lidx, d_off, g, axis);
     TG_LAUNCH_CHECK(id);
-    if (!exec) {
+    if (!use_graph) {
+        enqueue_core(st);
+    } else {
+        if (!exec) {
         // the sequence reads only buffers owned here, so one capture serves every later solve
         hipStream_t cap;
         TG_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), id);
@@ -707,6 +734,7 @@ void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t 
         TG_HIP(hipStreamDestroy(cap), id);
     }
     TG_HIP(hipGraphLaunch(exec, st), id);
+    }
     k_unpack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(xx, x, nodeMask, lidx, d_off, g, axis);
     TG_LAUNCH_CHECK(id);
 }

@@ -953,8 +953,7 @@ int tg_solver_last_timing(const tg_solver *s, double *ms_total) {
 
 extern "C" {
 
-// Average duration of one K*u kernel launch at `level` (precision 0: FP64, 1: FP32 V-cycle copy),
-// timed with HIP events around `reps` back-to-back launches on the solver's stream.
+// Launch choices of the solver (TG_TUNE_* keys); no effect on results.
 int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
     return guard("tg_solver_set_tuning:Input", [&] {
         const char *id = "tg_solver_set_tuning:Input";
@@ -963,6 +962,7 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
         else if (key == TG_TUNE_KFINE_ZCHUNK && value >= 1 && value <= 4096) S.kfine_zchunk = value;
         else if (key == TG_TUNE_KELEM_PARTS && (value == 4 || value == 6 || value == 8 || value == 12 || value == 24))
             S.kelem_parts = value;
+        else if (key == TG_TUNE_COARSE_GRAPH && (value == 0 || value == 1)) S.coarse.use_graph = value != 0;
         else if (key == TG_TUNE_TIME_KFINE && value >= 0 && value <= 2) {
             S.time_fine = value;
             S.tused = 0;
@@ -1001,6 +1001,8 @@ int tg_solver_kfine_timing(tg_solver *s, int32_t precision, int32_t epilogue, in
     });
 }
 
+// Average duration of one K*u kernel launch at `level` (precision 0: FP64, 1: FP32 V-cycle copy),
+// timed with HIP events around `reps` back-to-back launches on the solver's stream.
 int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t reps, double *ms_per_launch) {
     return guard("tg_solver_time_kbyu:Input", [&] {
         const char *id = "tg_solver_time_kbyu:Input";

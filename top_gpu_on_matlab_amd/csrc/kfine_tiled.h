@@ -6,8 +6,9 @@
 // trilinear function, evaluated at the vertices).  W^T W = 8 I, so for the 24x24 element matrix
 //     K0 = T KW T^T,   T = W (x) I_3,   KW = T^T K0 T / 64.
 // For any isotropic Hex8 matrix KW has 45 nonzeros of 576 (the constant mode carries no energy and
-// the cube's symmetries decouple most mode pairs), so f_e = E_e K0 u_e costs two 8-point butterflies
-// per component (72 adds each way), 45 FMAs and 21 scalings instead of 576 FMAs.
+// the cube's symmetries decouple most mode pairs), so f_e = E_e K0 u_e costs an 8-point butterfly per
+// component (72 adds), 45 FMAs, 21 scalings and the inverse butterflies (below: 24 adds per element
+// for the z stage and 24 per face for the y and x stages) instead of 576 FMAs.
 //
 // Tiling.  A workgroup owns a KT_TY x KT_TX column of nodes (y, x) over a chunk of z node planes
 // and marches through the element layers between them.  Per element layer:
@@ -15,11 +16,15 @@
 //      the previous layer) is committed to LDS, fixed DOFs zeroed (the zero columns of the
 //      BC-modified element matrices, TOP_GPU.m:704-720), and the plane after it is fetched;
 //   2. each of the 256 lanes computes the element vector of one element of the (KT_TY+1) x
-//      (KT_TX+1) layer and stores it in LDS;
-//   3. each owned node adds the 4 lower-vertex contributions of this layer to the upper-vertex
-//      contributions it kept from the previous layer (fixed order: deterministic), applies identity
-//      rows for fixed DOFs and the epilogue (K*u, r - K*u or the damped-Jacobi update), and
-//      writes its 3 values once; it then keeps this layer's 4 upper-vertex contributions.
+//      (KT_TX+1) layer in the Walsh basis and splits it by the inverse z stage into the values of its
+//      lower and upper faces (face Walsh basis).  The lower face is the face of the same (y, x) that
+//      the lane's element of the previous layer left as its upper face: the two are added, the sum is
+//      taken through the inverse y and x stages to its 4 vertex values, which go to LDS; the upper
+//      face is kept in registers for the next layer.  Each face is thus inverted once, not once per
+//      element on either side of it;
+//   3. each owned node sums the vertex values of the 4 faces around it (fixed order: deterministic),
+//      applies identity rows for fixed DOFs and the epilogue (K*u, r - K*u or the damped-Jacobi
+//      update), and writes its 3 values once.
 #pragma once
 
 #include "kernels.h"
@@ -149,10 +154,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
     constexpr bool J0 = EPI == EPI_RESID_JAC0;
     __shared__ T sU[2][3][KT_PN];
     __shared__ uint8_t sM[2][KT_PN];
-    // FP64 element vectors pass through LDS in two halves (lower, then upper vertices) so that four
-    // workgroups fit in a CU's LDS; FP32 ones in one pass.
-    constexpr bool SPLIT = sizeof(T) == 8;
-    __shared__ T sF[SPLIT ? 12 : 24][256];
+    // face values: the 4 vertex contributions x 3 components of each lane's face, [3 q + c][lane]
+    __shared__ T sF[12][256];
     const int t = threadIdx.x;
     int bid = blockIdx.x;
     const int tpc = kt.tilesY * kt.tilesX;
@@ -249,7 +252,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
     fetch(z0);
     fetchE(z0 - 1);
     fetchO(z0 - 1);
-    T part[3] = {0, 0, 0};  // upper-vertex contributions to the owned node of the next plane
+    // the upper face of this lane's element (the face of the same (y, x) in the next node plane), in
+    // the face's Walsh basis [3 i + c]: completed by the next layer's element before it is inverted
+    T kup[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     double dsum = 0;        // DOT: this lane's share, accumulated plane by plane
     int cur = 0;
     for (int ez = z0 - 1; ez < z1; ++ez) {
@@ -260,8 +265,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
         const uint8_t mk = owner ? sM[cur][(oy + 1) + KT_PY * (ox + 1)] : (uint8_t)0;
         fetch(ez + 2);  // past the chunk on the last layer: clamped, never committed
         fetchE(ez + 1);
-        // ---- element vector of this lane's element
-        T fv[24];
+        // ---- element vector of this lane's element, passed on as face values
         {
             const T Es = CK ? cs * Ee : Ee;
             T m[24];
@@ -287,34 +291,43 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
                         acc += (CK ? (T)KWALSH_NU03.v[i + 24 * j] : kw.v[KW_PATTERN.idx[i + 24 * j]]) * m[j];
                 gq[i] = acc * Es;
             }
+            // inverse z stage: modes q and q + 4 give the lower- and upper-face values of face mode q;
+            // the lower face (node plane ez) adds to what the element below left for it
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                T x[8];
+                T x[4];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) x[q] = gq[3 * q + c];
-                walsh_inv(x);
+                for (int q = 0; q < 4; ++q) {
+                    const T a = gq[3 * q + c], b = gq[3 * (q + 4) + c];
+                    x[q] = kup[3 * q + c] + (a - b);
+                    kup[3 * q + c] = a + b;
+                }
+                // inverse y and x stages of the face: vertex values, vertex q = vy + 2 vx
 #pragma unroll
-                for (int b = 0; b < 8; ++b) fv[3 * b + c] = x[vtx_q(b)];
+                for (int st = 1; st < 4; st <<= 1)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (!(q & st)) {
+                            const T a = x[q], b = x[q | st];
+                            x[q] = a - b;
+                            x[q | st] = a + b;
+                        }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) sF[3 * q + c][t] = x[q];
             }
         }
-#pragma unroll
-        for (int j = 0; j < (SPLIT ? 12 : 24); ++j) sF[j][t] = fv[j];
         lds_barrier();
-        // ---- owned node: complete plane ez, start plane ez + 1
-        T up[3] = {0, 0, 0};
+        // ---- owned node: complete plane ez from the 4 faces around it
         if (owner) {
-            T lo[3] = {part[0], part[1], part[2]};
+            T lo[3] = {0, 0, 0};
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx)
 #pragma unroll
                 for (int dy = 0; dy < 2; ++dy) {
                     const int te = (oy + dy) + KT_EY * (ox + dx);
-                    const int bl = vtx_of(1 - dy, 1 - dx, 0), bu = bl + 4;
+                    const int q = (1 - dy) + 2 * (1 - dx);
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        lo[d] += sF[3 * bl + d][te];
-                        if (!SPLIT) up[d] += sF[3 * bu + d][te];
-                    }
+                    for (int d = 0; d < 3; ++d) lo[d] += sF[3 * q + d][te];
                 }
             if (ez >= z0) {
                 T *o = out + 3 * NYX * ez + offO;
@@ -358,26 +371,6 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
                 }
             }
         }
-        if (SPLIT) {
-            lds_barrier();
-#pragma unroll
-            for (int j = 0; j < 12; ++j) sF[j][t] = fv[12 + j];
-            lds_barrier();
-            if (owner) {
-#pragma unroll
-                for (int dx = 0; dx < 2; ++dx)
-#pragma unroll
-                    for (int dy = 0; dy < 2; ++dy) {
-                        const int te = (oy + dy) + KT_EY * (ox + dx);
-                        const int bl = vtx_of(1 - dy, 1 - dx, 0);
-#pragma unroll
-                        for (int d = 0; d < 3; ++d) up[d] += sF[3 * bl + d][te];
-                    }
-            }
-        }
-        part[0] = up[0];
-        part[1] = up[1];
-        part[2] = up[2];
         fetchO(ez + 1);
         cur ^= 1;
     }

@@ -255,6 +255,30 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
     // the upper face of this lane's element (the face of the same (y, x) in the next node plane), in
     // the face's Walsh basis [3 i + c]: completed by the next layer's element before it is inverted
     T kup[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // FP32 (register room for it): the forward y and x butterflies of the element's lower face are
+    // those its upper face had in the previous layer, kept in klo [3 q + c]; each face is transformed
+    // once.  FP64 transforms both faces of every element (12 more registers would cost a wave).
+    constexpr bool KEEPF = sizeof(T) == 4;
+    T klo[KEEPF ? 12 : 1];
+    auto face_fwd = [&](int slot, T *f) {  // f[3 q + c] = face (ly, lx) of a staged plane, y and x stages
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            T x[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) x[q] = sU[slot][c][(ly + (q & 1)) + KT_PY * (lx + (q >> 1))];
+#pragma unroll
+            for (int st = 1; st < 4; st <<= 1)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (!(q & st)) {
+                        const T a = x[q], b = x[q | st];
+                        x[q] = a + b;
+                        x[q | st] = b - a;
+                    }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) f[3 * q + c] = x[q];
+        }
+    };
     double dsum = 0;        // DOT: this lane's share, accumulated plane by plane
     int cur = 0;
     for (int ez = z0 - 1; ez < z1; ++ez) {
@@ -269,17 +293,31 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
         {
             const T Es = CK ? cs * Ee : Ee;
             T m[24];
+            if (KEEPF) {
+                if (ez == z0 - 1) face_fwd(cur, klo);
+                T up[12];
+                face_fwd(cur ^ 1, up);
+                // forward z stage (as in walsh_fwd); the upper face is the next layer's lower face
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                T x[8];
-#pragma unroll
-                for (int b = 0; b < 8; ++b) {
-                    const int pn = (ly + vtx_y(b)) + KT_PY * (lx + vtx_x(b));
-                    x[vtx_q(b)] = sU[vtx_z(b) ? cur ^ 1 : cur][c][pn];
+                for (int i = 0; i < 12; ++i) {
+                    const T a = klo[i], b = up[i];
+                    m[i] = a + b;
+                    m[12 + i] = b - a;
+                    klo[i] = b;
                 }
-                walsh_fwd(x);
+            } else {
 #pragma unroll
-                for (int q = 0; q < 8; ++q) m[3 * q + c] = x[q];
+                for (int c = 0; c < 3; ++c) {
+                    T x[8];
+#pragma unroll
+                    for (int b = 0; b < 8; ++b) {
+                        const int pn = (ly + vtx_y(b)) + KT_PY * (lx + vtx_x(b));
+                        x[vtx_q(b)] = sU[vtx_z(b) ? cur ^ 1 : cur][c][pn];
+                    }
+                    walsh_fwd(x);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) m[3 * q + c] = x[q];
+                }
             }
             T gq[24];
 #pragma unroll

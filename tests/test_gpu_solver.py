@@ -229,3 +229,27 @@ def test_reassembly_is_bitwise_reproducible():
         c = s.coarse_solve(rc)
         assert torch.equal(c, c0), (k, float((c - c0).abs().max() / c0.abs().max()))
         assert torch.equal(s.vcycle(r), z0), k
+
+
+@pytest.mark.parametrize('super_element', [False, True])
+def test_mixed_vcycle_matches_fp64(super_element):
+    """The FP32 V-cycle (FP32 smoother, transfers and per-element matrices, the latter packed
+    symmetric; FP64 coarsest solve) against the FP64 V-cycle on the same system: they differ by FP32
+    rounding only, and the FP32 one stays symmetric to that level."""
+    import torch
+    from top_gpu_on_matlab_amd.model import cuboid_model
+    from top_gpu_on_matlab_amd.solver import Solver
+    shape, case = ((10, 20, 10), 2) if super_element else ((24, 48, 24), 1)
+    sv = {}
+    for p in ('fp64', 'mixed'):
+        s = Solver(cuboid_model(*shape, case, precision=p, super_element=super_element))
+        n = s.model.num_elements * (64 if super_element else 1)
+        s.set_modulus(torch.from_numpy(0.1 + np.random.default_rng(5).uniform(0, 1, n)).cuda())
+        s.assemble()
+        sv[p] = s
+    rng = np.random.default_rng(7)
+    r1, r2 = (torch.from_numpy(rng.standard_normal(sv['fp64'].ndof(1))).cuda() for _ in range(2))
+    z64, z32 = sv['fp64'].vcycle(r1), sv['mixed'].vcycle(r1)
+    assert float((z32 - z64).abs().max() / z64.abs().max()) < 2e-5
+    a, b = float(torch.dot(sv['mixed'].vcycle(r1), r2)), float(torch.dot(r1, sv['mixed'].vcycle(r2)))
+    assert abs(a - b) < 2e-5 * abs(a)

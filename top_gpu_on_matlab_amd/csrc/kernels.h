@@ -198,6 +198,61 @@ __global__ __launch_bounds__(256) void k_kelem_vec(const T *__restrict__ u, T *_
     }
 }
 
+// The FP32 V-cycle's copy of the per-element matrices, packed symmetric: entry (i, j), i <= j, of K_e
+// at Kp[(i + j (j + 1) / 2) * ldk + e], 300 of the 576 entries.  The element matrices are symmetric
+// (Galerkin products P' K P of symmetric BC-modified matrices, TOP_GPU.m:1729-1741; I_mat * E_sub on
+// the super-element level), so the upper triangle stands for both.  One thread per element reads
+// each stored entry once and adds it into both outputs it touches; outputs as in k_kelem_vec.
+constexpr int KSYM_N = 300;
+__host__ __device__ constexpr int ksym_index(int i, int j) { return i + j * (j + 1) / 2; }  // i <= j
+
+template <typename T, bool MASKED>
+__global__ __launch_bounds__(256) void k_kelem_sym(const T *__restrict__ u, T *__restrict__ f,
+                                                   const T *__restrict__ Kp, int64_t ldk,
+                                                   const uint8_t *__restrict__ eleMask,
+                                                   const uint8_t *__restrict__ mask, const Grid g) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.elems() || !eleMask[e]) return;
+    const int ey = (int)(e % g.EY);
+    const int64_t t = e / g.EY;
+    const int ex = (int)(t % g.EX), ez = (int)(t / g.EX);
+    T ue[24];
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const int64_t n = (ey + vtx_y(b)) + (int64_t)g.NY * ((ex + vtx_x(b)) + (int64_t)g.NX * (ez + vtx_z(b)));
+        const uint8_t mm = MASKED ? mask[n] : (uint8_t)0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ue[3 * b + d] = (mm & (1 << d)) ? (T)0 : u[3 * n + d];
+    }
+    const T *K = Kp + e;
+    T acc[24];
+#pragma unroll
+    for (int j = 0; j < 24; ++j) acc[j] = 0;
+#pragma unroll
+    for (int j = 0; j < 24; ++j)
+#pragma unroll
+        for (int i = 0; i <= j; ++i) {
+            const T k = K[(int64_t)ksym_index(i, j) * ldk];
+            acc[j] += k * ue[i];
+            if (i != j) acc[i] += k * ue[j];
+        }
+#pragma unroll
+    for (int j = 0; j < 24; ++j) f[(int64_t)j * ldk + e] = acc[j];
+}
+
+// Kp (packed upper triangle, T) from K (all 576 entries, column-major per element, S), entry-major.
+template <typename S, typename T>
+__global__ void k_pack_sym(const S *__restrict__ K, T *__restrict__ Kp, int64_t ldk) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= KSYM_N * ldk) return;
+    const int p = (int)(idx / ldk);
+    const int64_t e = idx - (int64_t)p * ldk;
+    int j = 0;
+    while (ksym_index(0, j + 1) <= p) ++j;
+    const int i = p - ksym_index(0, j);
+    Kp[idx] = (T)K[(int64_t)(i + 24 * j) * ldk + e];
+}
+
 template <typename T, int EPI, bool MASKED>
 __global__ __launch_bounds__(256) void k_kelem_gather(const T *__restrict__ f, int64_t ldk, T *__restrict__ out,
                                                       const T *__restrict__ u, const uint8_t *__restrict__ eleMask,

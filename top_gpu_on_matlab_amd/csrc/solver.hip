@@ -253,7 +253,7 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
             D.fv64.alloc(24 * D.ldk);
             D.Ks64.zero(st);
             if (prec == TG_PREC_MIXED) {
-                D.Ks32.alloc(576 * D.ldk);
+                D.Ks32.alloc(KSYM_N * D.ldk);  // packed symmetric (FP32 V-cycle)
                 D.fv32.alloc(24 * D.ldk);
             }
             // transfer coefficients on the finer grid (1 where a node is unused)
@@ -347,7 +347,7 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
         F.fv64.alloc(24 * F.ldk);
         F.Ks64.zero(st);
         if (prec == TG_PREC_MIXED) {
-            F.Ks32.alloc(576 * F.ldk);
+            F.Ks32.alloc(KSYM_N * F.ldk);  // packed symmetric (FP32 V-cycle)
             F.fv32.alloc(24 * F.ldk);
         }
     } else {
@@ -463,7 +463,7 @@ void Solver::assemble() {
             if (i < L - 1)
                 k_convert<double, float><<<grid_for(3 * D.g.nodes(), 256), 256, 0, st>>>(D.diag64, D.diag32, 3 * D.g.nodes());
             if ((i >= 1 && i < L - 1) || (i == 0 && super))
-                k_convert<double, float><<<grid_for(576 * D.ldk, 256), 256, 0, st>>>(D.Ks64, D.Ks32, 576 * D.ldk);
+                k_pack_sym<double, float><<<grid_for(KSYM_N * D.ldk, 256), 256, 0, st>>>(D.Ks64, D.Ks32, D.ldk);
         }
         if (!super) k_convert<double, float><<<grid_for(F.g.elems(), 256), 256, 0, st>>>(E64, E32, F.g.elems());
         TG_LAUNCH_CHECK(id);
@@ -567,13 +567,16 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
     } else {
         const T *Km = f64 ? (const T *)D.Ks64.p : (const T *)D.Ks32.p;
         T *fv = f64 ? (T *)D.fv64.p : (T *)D.fv32.p;
+        const unsigned gs = grid_for(D.g.elems(), 256);
         if (i == 0) {
-            launch_kelem_vec<T, true>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            if (f64) launch_kelem_vec<T, true>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            else k_kelem_sym<T, true><<<gs, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else k_kelem_gather<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
         } else {
-            launch_kelem_vec<T, false>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            if (f64) launch_kelem_vec<T, false>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            else k_kelem_sym<T, false><<<gs, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else k_kelem_gather<T, EPI_JACOBI, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);

@@ -29,9 +29,9 @@ METRIC = 'PCG solves/sec + K·u HBM GB/s, mixed vs FP64 at 1e-6 rel compliance e
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the FP64 finest K*u at 128x256x128 (bytes), from two rocprofv3 --pmc runs of
-# `bench.py --traffic-probe`: FETCH_SIZE 187,350 KiB x 2.0 + WRITE_SIZE 106,630 KiB x 1.0, the factors
+# `bench.py --traffic-probe`: FETCH_SIZE 183,906 KiB x 2.0 + WRITE_SIZE 106,650 KiB x 1.0, the factors
 # being what the counters read for a plain 2 GiB device copy (FETCH 1,048,576 KiB, WRITE 2,097,152 KiB)
-KFINE64_TRAFFIC = 492.9e6
+KFINE64_TRAFFIC = 485.8e6
 
 
 def log(*a):

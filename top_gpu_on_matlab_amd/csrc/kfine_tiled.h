@@ -141,16 +141,17 @@ __device__ __forceinline__ void walsh_inv(T x[8]) {
 // order): u.out for the plain product (p'Kp), r.out for the Jacobi epilogue (z'r).
 // EPI_RESID_JAC0: the V-cycle's pre-smoothing fused in: the operand is x = omega r ./ diag (formed
 // while staging; u is not read), x is written to xout and out = r - K x.
-// O64 (FP32 damped Jacobi with DOT only): the result goes to out64 in FP64 and the dot is taken
-// against rdot (the FP64 residual) -- the mixed V-cycle's last step, handing z straight to the FP64 CG.
-template <typename T, int EPI, bool CK, bool DOT, bool O64 = false>
+// R64 (FP32 damped Jacobi with DOT only): the dot is taken against rdot, the FP64 residual, instead of
+// the FP32 one -- the mixed V-cycle's last step, giving the FP64 CG its z.r (z itself stays FP32: its
+// widening to FP64 is exact and happens where the CG reads it).
+template <typename T, int EPI, bool CK, bool DOT, bool R64 = false>
 __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(const T *__restrict__ u, T *__restrict__ out,
                                                      const T *__restrict__ E, const uint8_t *__restrict__ mask,
                                                      const KWVals<T> kw, T cs, const Grid g, const KTiles kt,
                                                      const T *__restrict__ r, const T *__restrict__ diag, T omega,
                                                      double *__restrict__ partial, T *__restrict__ xout,
-                                                     double *__restrict__ out64, const double *__restrict__ rdot) {
-    static_assert(!O64 || (EPI == EPI_JACOBI && DOT && sizeof(T) == 4), "O64: FP32 Jacobi with the dot only");
+                                                     const double *__restrict__ rdot) {
+    static_assert(!R64 || (EPI == EPI_JACOBI && DOT && sizeof(T) == 4), "R64: FP32 Jacobi with the dot only");
     constexpr bool J0 = EPI == EPI_RESID_JAC0;
     __shared__ T sU[2][3][KT_PN];
     __shared__ uint8_t sM[2][KT_PN];
@@ -235,7 +236,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
     constexpr bool NEED_R = EPI != EPI_KU;
     constexpr bool NEED_D = EPI == EPI_JACOBI || J0;
     T oU[3] = {0, 0, 0}, oR[3] = {0, 0, 0}, oD[3] = {1, 1, 1};
-    double oW[3] = {0, 0, 0};  // O64: the FP64 residual for the dot
+    double oW[3] = {0, 0, 0};  // R64: the FP64 residual for the dot
     auto fetchO = [&](int z) {
         const int64_t b = 3 * NYX * zclamp(z, g.NZ) + offO;
 #pragma unroll
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
             if (NEED_U) oU[d] = u[b + d];
             if (NEED_R) oR[d] = r[b + d];
             if (NEED_D) oD[d] = diag[b + d];
-            if (O64) oW[d] = rdot[b + d];
+            if (R64) oW[d] = rdot[b + d];
         }
     };
 
@@ -383,19 +384,10 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
                         T v = (mk & (1 << d)) ? oU[d] : lo[d];
                         if (EPI == EPI_RESID) v = oR[d] - v;
                         if (EPI == EPI_JACOBI) v = oU[d] + (omega * (oR[d] - v)) / oD[d];
-                        if (O64) {
-                            out64[3 * NYX * ez + offO + d] = (double)v;
-                            dsum += (double)v * oW[d];
-                        } else {
-                            o[d] = v;
-                            if (DOT) dsum += (double)v * (double)(EPI == EPI_KU ? oU[d] : oR[d]);
-                        }
+                        o[d] = v;
+                        if (R64) dsum += (double)v * oW[d];
+                        else if (DOT) dsum += (double)v * (double)(EPI == EPI_KU ? oU[d] : oR[d]);
                     }
-                } else if (O64) {
-                    double *o6 = out64 + 3 * NYX * ez + offO;
-                    o6[0] = 0;
-                    o6[1] = 0;
-                    o6[2] = 0;
                 } else {
                     o[0] = 0;
                     o[1] = 0;

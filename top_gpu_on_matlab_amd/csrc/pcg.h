@@ -36,11 +36,18 @@ __global__ __launch_bounds__(RED_THREADS) static void k_pcg_update(double *__res
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// p = z + (x2 / x1) p  (TOP_GPU.m:652)
-__global__ static void k_pcg_direction(double *__restrict__ p, const double *__restrict__ z, int64_t n,
+// p = z + (x2 / x1) p  (TOP_GPU.m:652); Z = float: z as the mixed V-cycle left it (widening exact)
+template <typename Z>
+__global__ static void k_pcg_direction(double *__restrict__ p, const Z *__restrict__ z, int64_t n,
                                        const double *__restrict__ scal, int ix2, int ix1) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = z[i] + scal[ix2] / scal[ix1] * p[i];
+    if (i < n) p[i] = (double)z[i] + scal[ix2] / scal[ix1] * p[i];
+}
+
+// p = (double) z32 (the first search direction, from an FP32 preconditioner output)
+__global__ static void k_widen(double *__restrict__ p, const float *__restrict__ z, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (double)z[i];
 }
 
 // z64 = (double) z32 and partial sums of z64 . r64 (the mixed V-cycle's exit, fused with z'r)
@@ -72,6 +79,8 @@ struct PcgWork {
     double *h_scal = nullptr;
     float *r32 = nullptr;     // optional FP32 copy of r kept by the update (mixed V-cycle input)
     bool r32_fresh = false;   // r32 holds the current r
+    float *z32 = nullptr;     // optional FP32 preconditioner output (mixed V-cycle) ...
+    bool z32_fresh = false;   // ... set by ptv when it left z there instead of in z
 
     void init(int64_t count, hipStream_t s) {
         n = count;
@@ -116,8 +125,19 @@ struct PcgWork {
         }
         int sx1 = S_XA, sx2 = S_XB;
         r32_fresh = false;
-        if (!ptv(r.p, z.p, sx1)) dot(z, r, sx1);
-        TG_HIP(hipMemcpyAsync(p.p, z.p, n * sizeof(double), hipMemcpyDeviceToDevice, st), id);
+        z32_fresh = false;
+        // ptv may leave z in z32 (then with its dot fused); widen it wherever z itself is needed
+        auto precond = [&](int slot) {
+            const bool dotted = ptv(r.p, z.p, slot);
+            if (z32_fresh && !dotted) {
+                k_widen<<<grid_for(n, 256), 256, 0, st>>>(z.p, z32, n);
+                z32_fresh = false;
+            }
+            if (!dotted) dot(z, r, slot);
+        };
+        precond(sx1);
+        if (z32_fresh) k_widen<<<grid_for(n, 256), 256, 0, st>>>(p.p, z32, n);
+        else TG_HIP(hipMemcpyAsync(p.p, z.p, n * sizeof(double), hipMemcpyDeviceToDevice, st), id);
         int its = 0;
         double res = 0;
         for (its = 1; its <= maxIT; ++its) {
@@ -133,8 +153,9 @@ struct PcgWork {
             res = std::sqrt(read(S_RR)) / normB;
             if (!std::isfinite(res)) fail(TG_ERR_NUMERIC, id, "non-finite residual at iteration %d", its);
             if (res < tol) break;
-            if (!ptv(r.p, z.p, sx2)) dot(z, r, sx2);
-            k_pcg_direction<<<grid_for(n, 256), 256, 0, st>>>(p, z, n, scal, sx2, sx1);
+            precond(sx2);
+            if (z32_fresh) k_pcg_direction<float><<<grid_for(n, 256), 256, 0, st>>>(p, z32, n, scal, sx2, sx1);
+            else k_pcg_direction<double><<<grid_for(n, 256), 256, 0, st>>>(p, z, n, scal, sx2, sx1);
             TG_LAUNCH_CHECK(id);
             std::swap(sx1, sx2);
         }

@@ -81,11 +81,11 @@ struct Solver {
     template <typename T>
     void vcycle_t(const T *r0, T *z0);
     // r32_ready: rin32 already holds r0 rounded to FP32 (the CG update wrote it)
-    bool vcycle(const double *r0, double *z0, int dot_slot = -1, bool r32_ready = false);
+    bool vcycle(const double *r0, double *z0, int dot_slot = -1, bool r32_ready = false, bool z32_ok = false,
+                bool *z_in32 = nullptr);
     int vc_dot_slot = -1;  // finest post-smoothing also reduces z.r into pw.scal[vc_dot_slot]
     bool vc_dotted = false;
-    double *o64_z = nullptr;        // mixed: the FP32 finest post-smoothing writes z here in FP64 ...
-    const double *o64_r = nullptr;  // ... and reduces z.r against this FP64 residual
+    const double *r64_dot = nullptr;  // mixed: the FP32 finest post-smoothing reduces z.r against this FP64 r
     DevBuf<double> dotp;   // per-workgroup partial sums of the fused dots
     // compliance-gradient work space (tg_compliance_grad)
     DevBuf<double> adjF, adjXp, adjE, adjCe, adjG, adjPart, adjRes;

@@ -363,6 +363,7 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
         rin32.alloc(nv);
         zout32.alloc(nv);
         pw.r32 = rin32.p;
+        pw.z32 = zout32.p;
     }
     TG_HIP(hipEventCreate(&ev0), id);
     TG_HIP(hipEventCreate(&ev1), id);
@@ -489,13 +490,13 @@ static void launch_walsh_k(int epi, unsigned gt, hipStream_t st, const T *u, T *
                            const KWVals<T> &kw, T cs, const Grid &g, const KTiles &kt, const T *r, const T *d, T w,
                            double *partial, T *xout) {
     if (epi == EPI_KU)
-        k_kfine_walsh<T, EPI_KU, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, nullptr);
+        k_kfine_walsh<T, EPI_KU, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
     else if (epi == EPI_RESID)
-        k_kfine_walsh<T, EPI_RESID, CK, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, nullptr);
+        k_kfine_walsh<T, EPI_RESID, CK, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
     else if (epi == EPI_RESID_JAC0)
-        k_kfine_walsh<T, EPI_RESID_JAC0, CK, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, nullptr);
+        k_kfine_walsh<T, EPI_RESID_JAC0, CK, false><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
     else
-        k_kfine_walsh<T, EPI_JACOBI, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, nullptr);
+        k_kfine_walsh<T, EPI_JACOBI, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
 }
 
 template <typename T>
@@ -660,20 +661,20 @@ void Solver::vcycle_t(const T *r0, T *z0) {
         // the smoothed iterate goes to a second buffer (neighbours still read the old one);
         // on the finest level straight into the caller's output
         T *out = (i == 0) ? z0 : vec<T>(D, 2);
-        if (i == 0 && sizeof(T) == 4 && o64_z) {
-            // mixed: z = x + omega (r - K x) ./ d in FP32, stored in FP64, with z.r against the FP64 r
+        if (i == 0 && sizeof(T) == 4 && r64_dot) {
+            // mixed: z = x + omega (r - K x) ./ d in FP32, with z.r against the FP64 r
             const KTiles kt = ktiles_for(D.g, kfine_zchunk);
             const unsigned gt = (unsigned)kt.count();
             if ((int64_t)dotp.n < (int64_t)gt) dotp.alloc(gt);
             const float *x32 = (const float *)xx[0], *r32 = (const float *)rr[0], *d32 = D.diag32.p;
             if (const_k0)
                 k_kfine_walsh<float, EPI_JACOBI, true, true, true><<<gt, 256, 0, st>>>(
-                    x32, nullptr, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32, (float)omega,
-                    dotp.p, nullptr, o64_z, o64_r);
+                    x32, (float *)out, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32,
+                    (float)omega, dotp.p, nullptr, r64_dot);
             else
                 k_kfine_walsh<float, EPI_JACOBI, false, true, true><<<gt, 256, 0, st>>>(
-                    x32, nullptr, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32, (float)omega,
-                    dotp.p, nullptr, o64_z, o64_r);
+                    x32, (float *)out, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32,
+                    (float)omega, dotp.p, nullptr, r64_dot);
             k_reduce_final<<<1, RED_THREADS, 0, st>>>(dotp, (int)gt, pw.scal, vc_dot_slot);
             vc_dotted = true;
             continue;
@@ -690,22 +691,23 @@ void Solver::vcycle_t(const T *r0, T *z0) {
 }
 
 // z0 = M r0.  dot_slot >= 0: also leave z0.r0 in pw.scal[dot_slot] when a fused kernel can (returns
-// true), so PCG skips its separate dot.
-bool Solver::vcycle(const double *r0, double *z0, int dot_slot, bool r32_ready) {
+// true), so PCG skips its separate dot.  z32_ok (mixed): z may be left in FP32 in zout32 instead of
+// z0 (*z_in32 tells; widening it to FP64 is exact).
+bool Solver::vcycle(const double *r0, double *z0, int dot_slot, bool r32_ready, bool z32_ok, bool *z_in32) {
     vc_dot_slot = -1;
     vc_dotted = false;
+    if (z_in32) *z_in32 = false;
     if (prec == TG_PREC_MIXED) {
         DevLevel &F = lv[0];
         const int64_t nv = 3 * F.g.nodes();
         if (!r32_ready) k_convert<double, float><<<grid_for(nv, 256), 256, 0, st>>>(r0, rin32, nv);
-        if (dot_slot >= 0 && fine_tiled() && !adapted) {
+        if (dot_slot >= 0 && z32_ok && z_in32 && fine_tiled() && !adapted) {
             vc_dot_slot = dot_slot;
-            o64_z = z0;
-            o64_r = r0;
+            r64_dot = r0;
             vcycle_t<float>(rin32, zout32);
-            o64_z = nullptr;
-            o64_r = nullptr;
+            r64_dot = nullptr;
             vc_dot_slot = -1;
+            *z_in32 = true;
             TG_LAUNCH_CHECK("tg_solver:vcycle");
             return vc_dotted;
         }
@@ -739,8 +741,10 @@ void Solver::pcg(const double *b_c, double *y_c, double tol, int maxIT, int warm
     int its = 0;
     double res = 0;
     pw.run([&](const double *u, double *out, int slot) { return kbyu_level<double>(0, u, out, EPI_KU, nullptr, nullptr, slot); },
-           [&](const double *r, double *z, int slot) { return vcycle(r, z, slot, pw.r32_fresh); }, b, y, tol, maxIT, warm != 0, &its,
-           &res, "tg_solver_pcg:Numeric");
+           [&](const double *r, double *z, int slot) {
+               return vcycle(r, z, slot, pw.r32_fresh, pw.z32 != nullptr, &pw.z32_fresh);
+           },
+           b, y, tol, maxIT, warm != 0, &its, &res, "tg_solver_pcg:Numeric");
     k_pack<double, double><<<grid_for(F.Nn, 256), 256, 0, st>>>(y, y_c, F.nodFull0, F.Nn);
     TG_LAUNCH_CHECK(id);
     TG_HIP(hipEventRecord(ev1, st), id);

@@ -113,3 +113,34 @@ def test_entry_errors_on_device_inputs():
     with pytest.raises(TopGpuError) as ei:
         mex.scatter_accum3_inplace(_ga(np.ones(5, np.int32)), _ga(np.zeros((4, 3))), _ga(np.zeros((3, 3))))
     assert ei.value.identifier == 'scatter_accum3:shape'
+
+
+def test_entries_accept_empty_inputs():
+    """Zero elements (and zero keys) are valid inputs for all six entries: nothing is launched and the
+    accumulated outputs keep their values (the reference's kernels run zero blocks)."""
+    from top_gpu_on_matlab_amd import mex
+    Y0 = np.arange(30, dtype=np.float64).reshape(10, 3)
+    u = _ga(Y0)
+    gM = _ga(np.zeros((0, 24)))
+    mex.Gathering_inplace(gM, u, _ga(np.zeros((0, 8), np.int32)))
+    gY = _ga(Y0)
+    mex.Scattering_inplace(gY, _ga(np.zeros((0, 8), np.int32)), _ga(np.zeros((0, 24))))
+    assert np.array_equal(_np(gY), Y0)
+    gX = _ga(Y0)
+    mex.scatter_accum3_inplace(_ga(np.zeros(0, np.int32)), _ga(np.zeros((0, 3))), gX)
+    assert np.array_equal(_np(gX), Y0)
+    m = M.build_cuboid_model(8, 16, 8, opt_case=1)
+    lv1 = m.levels[0]
+    E = random_modulus(lv1.numElements, seed=5)
+    Pch, Psub = S.build_pch(4), S.build_pch(2)
+    empty_ks = _ga(np.zeros((24, 24, 0)))
+    mex.assembleKs_level2_inplace(empty_ks, _ga(np.zeros((0, 64), np.int32)), _ga(E),
+                                  _ga(lv1.Ke.reshape(-1, order='F')), _ga(Pch), _ga(m.mapUniqueKes),
+                                  _ga(m.uniqueKesFree), _ga(m.uniqueKesFixed))
+    mex.assembleKs_higherLevel_inplace(empty_ks, _ga(np.zeros((0, 8), np.int32)),
+                                       _ga(np.zeros((24, 24, 5))), _ga(Psub))
+    ms = M.build_cuboid_model(10, 20, 10, opt_case=2, super_element=True)
+    Esub = random_modulus(64 * ms.levels[0].numElements, seed=6).reshape(64, -1, order='F')
+    mex.assembleKs_level2_superEle_inplace(empty_ks, _ga(np.zeros((0, 64), np.int32)), _ga(ms.I_mat @ Esub),
+                                           _ga(Pch), _ga(ms.mapUniqueKes), _ga(Esub), _ga(ms.uniqueKesFree),
+                                           _ga(ms.uniqueKesFixed))

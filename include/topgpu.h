@@ -219,8 +219,9 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *   TG_TUNE_KFINE_ZCHUNK  node planes per workgroup of the tiled kernel (1..4096, default 12)
  *   TG_TUNE_TIME_KFINE    1 = bracket every tiled finest K*u launch with HIP events, 2 = only the
  *                         plain products (the PCG operator), 0 = stop; 1 and 2 clear the record
- *   TG_TUNE_KELEM_PARTS   threads per element of the per-element-matrix K*u (coarse levels and the
- *                         super-element finest level): 4, 6, 8, 12 or 24 (default 24)
+ *   TG_TUNE_KELEM_PARTS   threads per element of the FP64 per-element-matrix K*u (coarse levels and
+ *                         the super-element finest level): 4, 6, 8, 12 or 24 (default 24); the FP32
+ *                         V-cycle's packed symmetric product always uses one thread per element
  *   TG_TUNE_COARSE_GRAPH  1 = replay the coarsest solve's launch sequence as a captured HIP graph,
  *                         0 = launch it kernel by kernel (default: measured 1% faster per PCG iteration) */
 enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4,

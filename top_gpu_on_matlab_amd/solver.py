@@ -123,7 +123,7 @@ class Solver:
 
     def set_tuning(self, kfine_kernel=None, kfine_zchunk=None, kelem_parts=None, coarse_graph=None):
         """Finest-level K*u launch choice: kernel 'tiled' (default) or 'gather'; z planes per tile;
-        threads per element of the per-element-matrix K*u; coarsest solve replayed as a captured HIP
+        threads per element of the FP64 per-element-matrix K*u; coarsest solve replayed as a captured HIP
         graph (True) or launched kernel by kernel (False, default)."""
         from ._lib import TG_TUNE_COARSE_GRAPH, TG_TUNE_KELEM_PARTS, TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK
         if coarse_graph is not None:

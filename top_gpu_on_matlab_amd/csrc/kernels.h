@@ -360,6 +360,15 @@ __global__ __launch_bounds__(256) void k_restrict(const T *__restrict__ rf, T *_
     }
 }
 
+// Workgroups go to the 8 XCDs of the MI355X round-robin by blockIdx.x (workgroup 8 k + x is the k-th
+// one XCD x runs).  This renumbering gives XCD x a contiguous range of [0, n), so neighbouring tiles,
+// which share halo data, meet in the same L2.  A bijection of [0, n) for any n.
+constexpr int N_XCDS = 8;
+__device__ __forceinline__ int xcd_contiguous(int b, int n) {
+    const int q = n / N_XCDS, r = n % N_XCDS, x = b % N_XCDS, k = b / N_XCDS;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 // Restriction, element-centric and LDS-tiled (same sums as k_restrict, in separable order).  A
 // workgroup owns RT_E x RT_E coarse elements (y, x) over RT_Z coarse layers and marches through the
 // fine node planes of that box: each plane of v = r_f ./ coeff is staged in LDS (the next one is
@@ -380,7 +389,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_restrict_tiled(const T *__restri
     __shared__ T sV[NV];
     const int t = threadIdx.x;
     const int tilesY = (gc.EY + RT_E - 1) / RT_E, tilesX = (gc.EX + RT_E - 1) / RT_E;
-    int bid = blockIdx.x;
+    int bid = xcd_contiguous((int)blockIdx.x, (int)gridDim.x);
     const int tz = bid / (tilesY * tilesX);
     bid -= tz * tilesY * tilesX;
     const int e0y = (bid % tilesY) * RT_E, e0x = (bid / tilesY) * RT_E, e0z = tz * RT_Z;

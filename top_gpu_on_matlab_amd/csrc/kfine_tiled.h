@@ -137,8 +137,8 @@ __device__ __forceinline__ void walsh_inv(T x[8]) {
 
 // CK: KW is the compile-time unit matrix for nu = 0.3 and each element is scaled by cs * E_e;
 // otherwise KW (cellSize included) comes from kw and cs is ignored.
-// DOT: also write this workgroup's share of a CG dot product to partial[blockIdx.x] (FP64, fixed
-// order): u.out for the plain product (p'Kp), r.out for the Jacobi epilogue (z'r).
+// DOT: also write this workgroup's share of a CG dot product to partial[tile] (FP64, fixed order):
+// u.out for the plain product (p'Kp), r.out for the Jacobi epilogue (z'r).
 // EPI_RESID_JAC0: the V-cycle's pre-smoothing fused in: the operand is x = omega r ./ diag (formed
 // while staging; u is not read), x is written to xout and out = r - K x.
 // R64 (FP32 damped Jacobi with DOT only): the dot is taken against rdot, the FP64 residual, instead of
@@ -158,7 +158,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
     // face values: the 4 vertex contributions x 3 components of each lane's face, [3 q + c][lane]
     __shared__ T sF[12][256];
     const int t = threadIdx.x;
-    int bid = blockIdx.x;
+    // XCD-aware tile order: each XCD takes a contiguous range of tiles (xcd_contiguous)
+    int bid = xcd_contiguous((int)blockIdx.x, (int)gridDim.x);
+    const int tile = bid;  // the DOT partial goes to partial[tile]: the sum's order follows the tiles
     const int tpc = kt.tilesY * kt.tilesX;
     const int tz = bid / tpc;
     bid -= tz * tpc;
@@ -414,7 +416,7 @@ __global__ __launch_bounds__(256, sizeof(T) == 8 ? 3 : 4) void k_kfine_walsh(con
             if (t < w) red[t] += red[t + w];
             lds_barrier();
         }
-        if (t == 0) partial[blockIdx.x] = red[0];
+        if (t == 0) partial[tile] = red[0];
     }
 }
 

@@ -570,13 +570,13 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
         T *fv = f64 ? (T *)D.fv64.p : (T *)D.fv32.p;
         const unsigned gs = grid_for(D.g.elems(), 256);
         if (i == 0) {
-            if (f64) launch_kelem_vec<T, true>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            if constexpr (sizeof(T) == 8) launch_kelem_vec<T, true>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             else k_kelem_sym<T, true><<<gs, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else k_kelem_gather<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
         } else {
-            if (f64) launch_kelem_vec<T, false>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            if constexpr (sizeof(T) == 8) launch_kelem_vec<T, false>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             else k_kelem_sym<T, false><<<gs, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);

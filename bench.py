@@ -262,7 +262,7 @@ def main():
         'mixed_over_fp64': res['mixed']['solves_per_s'] / res['fp64']['solves_per_s'],
         'rel_compliance_err_mixed_vs_fp64': {'max': max(errs), 'per_iteration': errs},
         'kbyu': kst,
-        'roofline': {'kernel': 'finest-level K*u, FP64, PCG operator with the fused p.Kp (k_kfine_walsh<double,0,true,true>), '
+        'roofline': {'kernel': 'finest-level K*u, FP64, PCG operator with the fused p.Kp (k_kfine_walsh<double,0,true,true,false>), '
                                'launches of the mixed solver in the timed steps', 'bound': 'hbm',
                      'achieved': gbs64, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': gbs64 / HBM_PEAK_GBS,
                      'traffic': KFINE64_TRAFFIC, 'bytes_alg': b64, 'avg_launch_us': us64, 'launches': kst['fp64_ku']['launches'],

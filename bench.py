@@ -161,8 +161,14 @@ def traffic_probe(shape):
         b.copy_(a)
     for _ in range(2):
         torch.mul(a, 1.0, out=b)
+    # read-only (a reduction: 2 GiB in, 8 bytes out) and write-only (a fill: 2 GiB out) kernels settle
+    # each counter's scale on its own
+    for _ in range(2):
+        float(a.sum())
+    for _ in range(2):
+        b.fill_(2.0)
     torch.cuda.synchronize()
-    log('copies done (2 GiB read + 2 GiB write each)')
+    log('copies done (2 GiB read + 2 GiB write each), reductions (2 GiB read), fills (2 GiB write)')
 
 
 def main():

@@ -255,8 +255,9 @@ int tg_simp_modulus(const double *d_x, int64_t n, double Emin, double E0, double
 int tg_sensitivity(const double *d_x, const double *d_ce, int64_t n, int32_t per, double Emin, double E0,
                    double penal, double cSolid, double *d_dc, void *stream);
 
-enum { TG_EW_MUL = 0, TG_EW_DIV_FLOOR = 1 };
-/* out = a .* b (TG_EW_MUL) or a ./ max(scalar, b) (TG_EW_DIV_FLOOR)  (ft = 1 filtering, TOP_GPU.m:214). */
+enum { TG_EW_MUL = 0, TG_EW_DIV_FLOOR = 1, TG_EW_SET_WHERE = 2 };
+/* out = a .* b (TG_EW_MUL) or a ./ max(scalar, b) (TG_EW_DIV_FLOOR)  (ft = 1 filtering, TOP_GPU.m:214);
+ * out = (b ~= 0) ? scalar : a (TG_EW_SET_WHERE: xPhys(passiveElements) = 1, TOP_GPU.m:270, 273). */
 int tg_elementwise(int32_t op, const double *d_a, const double *d_b, double scalar, int64_t n, double *d_out,
                    void *stream);
 
@@ -266,10 +267,13 @@ int tg_reduce(int32_t op, const double *d_a, const double *d_b, int64_t n, doubl
 
 /* Optimality-criteria update with bisection on the Lagrange multiplier (TOP_GPU.m:233-253):
  * l1 = 0, l2 = 1e9, move; xnew = max(0,max(x-move,min(1,min(x+move,x.*sqrt(-dc./dv/lmid))))),
- * gt = fval + mean(xnew - x).  Writes d_xnew, *change = max|xnew - x|, *bisections = steps run
- * (a multiple of 16; trailing steps are no-ops).  Synchronises. */
-int tg_oc_update(const double *d_x, const double *d_dc, const double *d_dv, int64_t n, double fval, double move,
-                 double *d_xnew, double *change, int32_t *bisections, void *stream);
+ * xnew(passive) = 1, gt = fval + mean(xnew - x).  d_passive: NULL (no passive elements) or [n] with a
+ * nonzero entry where xnew is held at 1 in every bisection step (TOP_GPU.m:239, 247; in super-element
+ * mode the caller marks sub-voxel row 4 of each passive element, as xnew(Num_, passiveElements) does).
+ * Writes d_xnew, *change = max|xnew - x|, *bisections = steps run (a multiple of 16; trailing steps are
+ * no-ops).  Synchronises. */
+int tg_oc_update(const double *d_x, const double *d_dc, const double *d_dv, const double *d_passive, int64_t n,
+                 double fval, double move, double *d_xnew, double *change, int32_t *bisections, void *stream);
 
 /* PDE (Helmholtz) density filter on the finest grid (TOP_GPU.m:880-1031). */
 typedef struct tg_filter tg_filter;

@@ -123,11 +123,17 @@ class ConvolutionFilter:
 
 
 def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0, penal=3.0,
-            tol=1e-3, maxIT=800, verbose=False, x_init=None, weight_factor_jacobi=0.6):
-    """Optimization driver TOP_GPU (TOP_GPU.m:93-292), FP64, no passive elements.
+            tol=1e-3, maxIT=800, verbose=False, x_init=None, weight_factor_jacobi=0.6, passive=None):
+    """Optimization driver TOP_GPU (TOP_GPU.m:93-292), FP64.
+
+    passive: 0-based solid element ids held at density 1 (passiveElements_, TOP_GPU.m:123-125).  As in
+    the reference they are set inside every bisection step (so they enter gt, TOP_GPU.m:239, 247) and
+    again after the density filter (:270, :273); in super-element mode only sub-voxel row Num_ = 4
+    of each passive element is set (xnew(Num_, passiveElements), TOP_GPU.m:247, 273).
 
     Returns dict with cHist, volHist, sharpHist, consHist, lssIts, cSolid, xPhys, U, times.
     """
+    pas = np.asarray(passive if passive is not None else [], dtype=np.int64)
     sv = S.Solver(model, weight_factor_jacobi=weight_factor_jacobi)
     lv1 = model.levels[0]
     Ne = lv1.numElements
@@ -205,6 +211,11 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
         while (l2 - l1) / (l1 + l2) > 1e-6:
             lmid = 0.5 * (l2 + l1)
             xnew = np.maximum(0, np.maximum(x - move, np.minimum(1, np.minimum(x + move, x * np.sqrt(-dc / dv / lmid)))))
+            if pas.size:
+                if not se:
+                    xnew[pas] = 1.0
+                else:
+                    xnew[num - 1, pas] = 1.0
             gt = fval + float(np.mean(xnew - x))
             margin = min(margin, abs(gt))
             if gt > 0:
@@ -218,6 +229,12 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
             xPhys = xnew
         elif ft == 2:
             xPhys = filt(xnew) if not se else filt.forward(xnew)
+        if pas.size:
+            xPhys = xPhys.copy()
+            if not se:
+                xPhys[pas] = 1.0
+            else:
+                xPhys[num - 1, pas] = 1.0
         if not se:
             sharp = 4 * float(np.sum(xPhys * (1 - xPhys))) / Ne
         else:

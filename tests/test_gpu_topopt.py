@@ -148,3 +148,47 @@ def test_c1_fifty_iteration_history_against_golden():
         assert dev.max() < max(10 * spread.max(), 1e-10), key
         if key == 'cHist':
             assert got['lssIts'][:K] == g['lssIts'][:K]
+
+
+@pytest.mark.parametrize('se', [False, True], ids=['topvoxel-ft2', 'mbb-super'])
+def test_passive_elements_match_oracle(se, tmp_path):
+    """Passive elements from a .TopVoxel file (standard mode) and on a super-element model (only
+    sub-voxel row 4 is held at 1, xnew(Num_, passiveElements), TOP_GPU.m:247, 273) against the oracle."""
+    from top_gpu_on_matlab_amd.model import cuboid_model
+    from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
+    print('\n[passive se=%s]' % se, flush=True)
+    if not se:
+        from tests.test_io_cpu import _conditions
+        from top_gpu_on_matlab_amd.topvoxel import write_topvoxel
+        nely, nelx, nelz = 8, 16, 8
+        fix, ld = _conditions(nely, nelx, nelz)
+        file_ids = [1 + y + nely * (x + nelx * z) for z in (2, 3) for x in (7, 8) for y in (0, 1)]
+        path = os.path.join(tmp_path, 'passive.TopVoxel')
+        write_topvoxel(path, np.ones((nely, nelx, nelz), bool), fix, ld, passive=file_ids)
+        om = M.build_file_model(nelx, nely, nelz, np.arange(1, nely * nelx * nelz + 1), fix, ld)
+        e0 = np.array(file_ids) - 1
+        full = e0 % nely + om.levels[0].resY * ((e0 // nely) % nelx + om.levels[0].resX * (e0 // (nely * nelx)))
+        pas = np.sort(om.levels[0].eleMapForward[full].astype(np.int64) - 1)
+        ref = T.run_top(om, V0=0.12, rMin=math.sqrt(3), nLoop=4, ft=2, passive=pas)
+        got = TOP_GPU(path, V0=0.12, rMin=math.sqrt(3), nLoop=4, ft=2)
+        assert list(got['model'].passive) == list(pas)
+        xp = got['xPhys'].cpu().numpy()
+        assert np.all(xp[pas] == 1.0)
+        refx = np.asarray(ref['xPhys'])
+    else:
+        shape = (10, 20, 10)
+        om = M.build_cuboid_model(*shape, opt_case=2, super_element=True)
+        gm = cuboid_model(*shape, 2, super_element=True)
+        pas = np.array([0, 17, 555, 1999])
+        gm.passive = pas
+        ref = T.run_top(om, V0=0.12, rMin=math.sqrt(3), nLoop=3, ft=2, passive=pas)
+        got = TOP_GPU(np.ones(shape, bool), optCase=2, V0=0.12, nLoop=3, ft=2, super_element=1, model=gm)
+        xp = got['xPhys'].cpu().numpy().reshape(64, -1, order='F')
+        assert np.all(xp[3, pas] == 1.0)
+        refx = np.asarray(ref['xPhys'])
+        xp = xp.reshape(-1, order='F')
+        refx = refx.reshape(-1, order='F')
+    assert got['lssIts'] == ref['lssIts']
+    for k in ('cHist', 'volHist', 'sharpHist', 'consHist'):
+        assert _rel(got[k], ref[k]) < 1e-10, (k, got[k], ref[k])
+    assert _rel(xp, refx) < 1e-9

@@ -285,3 +285,22 @@ def test_vectorised_assembly_matches_literal_loop():
     b.assemble(E, fast=True)
     for k, lv in zip(ks_lit, m.levels[1:]):
         assert np.abs(lv.Ks - k).max() < 1e-13 * np.abs(k).max()
+
+
+@pytest.mark.parametrize('se', [False, True])
+def test_passive_elements_held_at_one(se):
+    """Passive elements (TOP_GPU.m:239, 247, 270, 273): xPhys is 1 there after every iteration (in
+    super-element mode only sub-voxel row 4, as xnew(Num_, passiveElements) writes), the constraint
+    still holds on the full design, and the history differs from the run without them."""
+    shape, case = ((8, 16, 8), 1) if not se else ((8, 8, 8), 2)
+    m = M.build_cuboid_model(*shape, opt_case=case, super_element=se)
+    pas = np.array([0, 9, 130, 300])
+    a = T.run_top(m, V0=0.12, nLoop=2, ft=2, passive=pas)
+    b = T.run_top(m, V0=0.12, nLoop=2, ft=2)
+    xp = np.asarray(a['xPhys'])
+    if se:
+        assert np.all(xp[3, pas] == 1.0) and np.all(xp[:3, pas] < 1.0)
+    else:
+        assert np.all(xp[pas] == 1.0)
+        assert np.all(np.asarray(b['xPhys'])[pas] < 1.0)
+    assert np.abs(np.array(a['cHist']) - np.array(b['cHist'])).max() > 1e-6 * abs(b['cHist'][-1])

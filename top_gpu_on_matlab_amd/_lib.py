@@ -97,7 +97,7 @@ SIGNATURES = {
     'tg_sensitivity': (ctypes.c_int, [_VP, _VP, _I64, _I32, _D, _D, _D, _D, _VP, _VP]),
     'tg_elementwise': (ctypes.c_int, [_I32, _VP, _VP, _D, _I64, _VP, _VP]),
     'tg_reduce': (ctypes.c_int, [_I32, _VP, _VP, _I64, ctypes.POINTER(_D), _VP]),
-    'tg_oc_update': (ctypes.c_int, [_VP, _VP, _VP, _I64, _D, _D, _VP, ctypes.POINTER(_D), ctypes.POINTER(_I32), _VP]),
+    'tg_oc_update': (ctypes.c_int, [_VP, _VP, _VP, _VP, _I64, _D, _D, _VP, ctypes.POINTER(_D), ctypes.POINTER(_I32), _VP]),
     'tg_filter_create': (ctypes.c_int, [_VP, _D, _VP, ctypes.POINTER(_VP)]),
     'tg_filter_destroy': (None, [_VP]),
     'tg_filter_apply': (ctypes.c_int, [_VP, _VP, _VP, _D, _I32, ctypes.POINTER(_I32)]),
@@ -106,7 +106,7 @@ SIGNATURES = {
     'tg_convfilter_forward': (ctypes.c_int, [_VP, _VP, _VP]),
     'tg_convfilter_adjoint': (ctypes.c_int, [_VP, _VP, _VP]),
 }
-TG_EW_MUL, TG_EW_DIV_FLOOR = 0, 1
+TG_EW_MUL, TG_EW_DIV_FLOOR, TG_EW_SET_WHERE = 0, 1, 2
 TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK, TG_TUNE_TIME_KFINE, TG_TUNE_KELEM_PARTS, TG_TUNE_COARSE_GRAPH = 1, 2, 3, 4, 5
 TG_RED_SUM, TG_RED_DOT, TG_RED_MAXABSDIFF, TG_RED_SHARPNESS = 0, 1, 2, 3
 

@@ -101,6 +101,13 @@ __global__ void k_div_floor(const double *__restrict__ a, const double *__restri
     if (i < n) c[i] = a[i] / fmax(fl, x[i]);
 }
 
+// c = (b != 0) ? v : a  (xPhys(passiveElements) = 1 after the density filter, TOP_GPU.m:270, 273)
+__global__ void k_set_where(const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c, double v,
+                            int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = b[i] != 0.0 ? v : a[i];
+}
+
 // ---- reductions ------------------------------------------------------------------------------
 enum { R_SUM = 0, R_DOT = 1, R_MAXABSDIFF = 2, R_SHARP = 3 };
 
@@ -151,15 +158,25 @@ __device__ inline double oc_xnew(double x, double dc, double dv, double lmid, do
     return fmax(0.0, fmax(x - move, fmin(1.0, fmin(x + move, x * sqrt(-dc / dv / lmid)))));
 }
 
+// passive (may be null): entries with a nonzero indicator take xnew = 1 inside every bisection step, so
+// they enter gt (xnew(passiveElements) = 1.0, TOP_GPU.m:239, 247)
+__device__ inline double oc_xnew_p(const double *__restrict__ x, const double *__restrict__ dc,
+                                   const double *__restrict__ dv, const double *__restrict__ passive, int64_t i,
+                                   double lmid, double move) {
+    if (passive && passive[i] != 0.0) return 1.0;
+    return oc_xnew(x[i], dc[i], dv[i], lmid, move);
+}
+
 __global__ __launch_bounds__(RED_THREADS) void k_oc_step(const double *__restrict__ x, const double *__restrict__ dc,
-                                                         const double *__restrict__ dv, int64_t n,
+                                                         const double *__restrict__ dv,
+                                                         const double *__restrict__ passive, int64_t n,
                                                          const double *__restrict__ scal, double *__restrict__ partial) {
     if (!oc_open(scal)) return;
     const double lmid = 0.5 * (scal[1] + scal[0]), move = scal[4];
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
     double s = 0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s += oc_xnew(x[i], dc[i], dv[i], lmid, move) - x[i];
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s += oc_xnew_p(x, dc, dv, passive, i, lmid, move) - x[i];
     s = block_sum<double>(s);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
@@ -180,9 +197,10 @@ __global__ __launch_bounds__(RED_THREADS) void k_oc_bisect(const double *__restr
 }
 
 __global__ void k_oc_apply(const double *__restrict__ x, const double *__restrict__ dc, const double *__restrict__ dv,
-                           double *__restrict__ xnew, int64_t n, const double *__restrict__ scal) {
+                           const double *__restrict__ passive, double *__restrict__ xnew, int64_t n,
+                           const double *__restrict__ scal) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) xnew[i] = oc_xnew(x[i], dc[i], dv[i], scal[2], scal[4]);
+    if (i < n) xnew[i] = oc_xnew_p(x, dc, dv, passive, i, scal[2], scal[4]);
 }
 
 // ---- PDE filter ------------------------------------------------------------------------------
@@ -563,6 +581,7 @@ int tg_elementwise(int32_t op, const double *d_a, const double *d_b, double scal
         if (!n) return;
         if (op == TG_EW_MUL) k_mul<<<grid_for(n, 256), 256, 0, st>>>(d_a, d_b, d_out, n);
         else if (op == TG_EW_DIV_FLOOR) k_div_floor<<<grid_for(n, 256), 256, 0, st>>>(d_a, d_b, d_out, scalar, n);
+        else if (op == TG_EW_SET_WHERE) k_set_where<<<grid_for(n, 256), 256, 0, st>>>(d_a, d_b, d_out, scalar, n);
         else fail(TG_ERR_ARGS, "tg_elementwise:Input", "unknown op %d", op);
         TG_LAUNCH_CHECK("tg_elementwise:HIP");
     });
@@ -585,8 +604,8 @@ int tg_reduce(int32_t op, const double *d_a, const double *d_b, int64_t n, doubl
     });
 }
 
-int tg_oc_update(const double *d_x, const double *d_dc, const double *d_dv, int64_t n, double fval, double move,
-                 double *d_xnew, double *change, int32_t *bisections, void *stream) {
+int tg_oc_update(const double *d_x, const double *d_dc, const double *d_dv, const double *d_passive, int64_t n,
+                 double fval, double move, double *d_xnew, double *change, int32_t *bisections, void *stream) {
     return guard("tg_oc_update:Input", [&] {
         const char *id = "tg_oc_update:Input";
         if (!d_x || !d_dc || !d_dv || !d_xnew || n < 1) fail(TG_ERR_ARGS, id, "bad arguments");
@@ -602,7 +621,7 @@ int tg_oc_update(const double *d_x, const double *d_dc, const double *d_dv, int6
             // (l2-l1)/(l1+l2) halves each step from 1: at most ~100 steps; check every 16
             while (open) {
                 for (int k = 0; k < 16; ++k) {
-                    k_oc_step<<<RED_BLOCKS, RED_THREADS, 0, st>>>(d_x, d_dc, d_dv, n, scal, partial);
+                    k_oc_step<<<RED_BLOCKS, RED_THREADS, 0, st>>>(d_x, d_dc, d_dv, d_passive, n, scal, partial);
                     k_oc_bisect<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal);
                 }
                 TG_LAUNCH_CHECK(id);
@@ -612,7 +631,7 @@ int tg_oc_update(const double *d_x, const double *d_dc, const double *d_dv, int6
                 open = (h[1] - h[0]) / (h[0] + h[1]) > 1e-6;
                 if (steps > 400) fail(TG_ERR_NUMERIC, "tg_oc_update:Numeric", "bisection did not converge");
             }
-            k_oc_apply<<<grid_for(n, 256), 256, 0, st>>>(d_x, d_dc, d_dv, d_xnew, n, scal);
+            k_oc_apply<<<grid_for(n, 256), 256, 0, st>>>(d_x, d_dc, d_dv, d_passive, d_xnew, n, scal);
             TG_LAUNCH_CHECK(id);
             if (change) *change = reduce_dev(R_MAXABSDIFF, d_xnew, d_x, n, partial, res, h, st);
             if (bisections) *bisections = steps;

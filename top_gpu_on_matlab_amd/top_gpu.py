@@ -111,13 +111,30 @@ def elementwise(op, a, b, scalar=0.0):
     return out
 
 
-def oc_update(x, dc, dv, fval, move=0.2):
+def oc_update(x, dc, dv, fval, move=0.2, passive=None):
+    """passive: None or a float64 indicator like x (nonzero: xnew held at 1, TOP_GPU.m:239, 247)."""
     xnew = torch.empty_like(x)
     change = ctypes.c_double()
     steps = ctypes.c_int32()
-    check(lib().tg_oc_update(_p(x), _p(dc), _p(dv), x.numel(), float(fval), float(move), _p(xnew),
-                             ctypes.byref(change), ctypes.byref(steps), _stream()))
+    check(lib().tg_oc_update(_p(x), _p(dc), _p(dv), _p(passive) if passive is not None else None, x.numel(),
+                             float(fval), float(move), _p(xnew), ctypes.byref(change), ctypes.byref(steps),
+                             _stream()))
     return xnew, change.value
+
+
+def passive_indicator(model, shape, super_element, num=4):
+    """Indicator of the passive entries of the design array (None when the model has none): element e
+    itself, or in super-element mode only sub-voxel row Num_ = 4 of element e (the reference's
+    xnew(Num_, passiveElements) and xPhys(Num_, passiveElements), TOP_GPU.m:247, 273)."""
+    pas = np.asarray(getattr(model, 'passive', ()), dtype=np.int64)
+    if pas.size == 0:
+        return None
+    ind = np.zeros(int(np.prod(shape)))
+    if super_element:
+        ind[num ** 3 * pas + (num - 1)] = 1.0
+    else:
+        ind[pas] = 1.0
+    return torch.from_numpy(ind).to('cuda')
 
 
 def unit_compliance(solver, U):
@@ -191,6 +208,7 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
     cSolid = reduce(_lib.TG_RED_DOT, E1, ce) if not se else reduce(_lib.TG_RED_SUM, ce)
     out = dict(cHist=[], volHist=[], sharpHist=[], consHist=[], lssIts=[], tHist=[], changeHist=[],
                cSolid=cSolid, pcg_ms=[], filter_its=[])
+    passive = passive_indicator(model, shape, se, num)
     loop, change, sharp = 0, 1.0, 1.0
     ones = torch.ones(shape, dtype=torch.float64, device=dev)
     while loop < nLoop and change > 0.0001 and sharp > 0.01:
@@ -235,7 +253,7 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
         torch.cuda.synchronize()
         t4 = time.perf_counter()
         fval = reduce(_lib.TG_RED_SUM, xPhys) / nel - V0
-        xnew, change = oc_update(x, dc, dv, fval, 0.2)
+        xnew, change = oc_update(x, dc, dv, fval, 0.2, passive)
         x = xnew
         t5 = time.perf_counter()
         if ft == 1:
@@ -246,6 +264,8 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
                 fits.append(filt.last_its)
             else:
                 xPhys = filt.forward(xnew)
+        if passive is not None:       # xPhys(passiveElements) = 1 (TOP_GPU.m:270, 273)
+            xPhys = elementwise(_lib.TG_EW_SET_WHERE, xPhys, passive, 1.0)
         sharp = 4 * reduce(_lib.TG_RED_SHARPNESS, xPhys) / nel
         t6 = time.perf_counter()
         out['cHist'].append(cDesign)

@@ -171,6 +171,89 @@ def traffic_probe(shape):
     log('copies done (2 GiB read + 2 GiB write each), reductions (2 GiB read), fills (2 GiB write)')
 
 
+def mex_timing(shape, reps):
+    """The six MEX entry points on the connectivity and transfer maps of the `shape` cantilever (the
+    super-element assembly on the 10x60x10 MBB model, its own configuration), FP64, inputs resident on
+    the device; HIP events around `reps` calls after one warm-up call.  The scatter entry points reuse
+    their cached incidence after the first call (the content hash of the index array is recomputed
+    every call and included)."""
+    import torch
+    from top_gpu_on_matlab_amd import mex
+    from top_gpu_on_matlab_amd.model import cuboid_model
+    from top_gpu_on_matlab_amd.mex import GpuArray
+
+    def ga(a):
+        a = np.asarray(a)
+        return GpuArray(torch.from_numpy(np.ascontiguousarray(a.reshape(-1, order='F'))).cuda(), a.shape)
+
+    def zeros(dims, dtype=torch.float64):
+        return GpuArray(torch.zeros(int(np.prod(dims)), dtype=dtype, device='cuda'), dims)
+
+    rng = np.random.default_rng(1)
+    m = cuboid_model(*shape, 1)
+    i1, i2 = m.level_info(1), m.level_info(2)
+    Ne, Nn, Nc, Nnc = i1['numElements'], i1['numNodes'], i2['numElements'], i2['numNodes']
+    eNod, eNod2 = ga(m.get_map(1, 'eNodMat')), ga(m.get_map(2, 'eNodMat'))
+    up2 = ga(m.get_map(2, 'elementUpwardMap'))
+    tm2 = m.get_map(2, 'transferMat')
+    nInt = i2['intermediateNumNodes']
+    u = ga(rng.uniform(-1, 1, (Nn, 3)))
+    uMat = zeros((Ne, 24))
+    E = ga(1e-6 + rng.random(Ne) ** 3)
+    Ke = m.get_table(1, 'Ke')
+    muk = ga(m.get_map(1, 'mapUniqueKes'))
+    uf, ux = ga(m.get_table(1, 'uniqueKesFree')), ga(m.get_table(1, 'uniqueKesFixed'))
+    Pch = ga(rng.standard_normal((24, 24, up2.dims[1])))
+    cases = {
+        'Gathering_inplace': (mex.Gathering_inplace, (uMat, u, eNod),
+                              'finest: uMat [%d x 24] <- u [%d x 3] via eNodMat' % (Ne, Nn)),
+        'Scattering_inplace': (mex.Scattering_inplace, (zeros((Nn, 3)), eNod, ga(rng.standard_normal((Ne, 24)))),
+                               'finest: Y [%d x 3] += uMat [%d x 24] via eNodMat' % (Nn, Ne)),
+        'scatter_accum3_inplace (restriction)': (
+            mex.scatter_accum3_inplace, (ga(m.get_map(2, 'eNodMat').reshape(-1, order='F')),
+                                         ga(rng.standard_normal((8 * Nc, 3))), zeros((Nnc, 3))),
+            'L1->L2: keys = level-2 eNodMat (N = %d) into M = %d coarse nodes' % (8 * Nc, Nnc)),
+        'scatter_accum3_inplace (prolongation)': (
+            mex.scatter_accum3_inplace, (ga(tm2.reshape(-1, order='F')), ga(rng.standard_normal((tm2.size, 3))),
+                                         zeros((nInt, 3))),
+            'L2->L1: keys = level-2 transferMat (N = %d) into M = %d intermediate nodes' % (tm2.size, nInt)),
+        'assembleKs_level2_inplace': (
+            mex.assembleKs_level2_inplace, (zeros((24, 24, Nc)), up2, E, ga(Ke.reshape(-1, order='F')), Pch, muk, uf, ux),
+            'Ks [24 x 24 x %d], 64 children each, %d boundary element matrices' % (Nc, uf.dims[1])),
+    }
+    if m.num_levels >= 3:
+        i3 = m.level_info(3)
+        up3 = ga(m.get_map(3, 'elementUpwardMap'))
+        cases['assembleKs_higherLevel_inplace'] = (
+            mex.assembleKs_higherLevel_inplace, (zeros((24, 24, i3['numElements'])), up3,
+                                                 ga(rng.standard_normal((24, 24, Nc))),
+                                                 ga(rng.standard_normal((24, 24, up3.dims[1])))),
+            'Ks [24 x 24 x %d] from KsPrev [24 x 24 x %d]' % (i3['numElements'], Nc))
+    ms = cuboid_model(10, 60, 10, 2, super_element=True)
+    s1, s2 = ms.level_info(1), ms.level_info(2)
+    sup = ga(ms.get_map(2, 'elementUpwardMap'))
+    cases['assembleKs_level2_superEle_inplace'] = (
+        mex.assembleKs_level2_superEle_inplace,
+        (zeros((24, 24, s2['numElements'])), sup, ga(rng.standard_normal((576, s1['numElements']))),
+         ga(rng.standard_normal((24, 24, sup.dims[1]))), ga(ms.get_map(1, 'mapUniqueKes')),
+         ga(rng.random((64, s1['numElements']))), ga(ms.get_table(1, 'uniqueKesFree')),
+         ga(ms.get_table(1, 'uniqueKesFixed'))),
+        '10x60x10 MBB super-element: Ks [24 x 24 x %d] from %d fine elements' % (s2['numElements'], s1['numElements']))
+    out = {}
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for name, (fn, args, what) in cases.items():
+        fn(*args)  # warm-up (the scatter entries build their incidence here)
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(reps):
+            fn(*args)
+        ev1.record()
+        torch.cuda.synchronize()
+        out[name] = {'ms_per_call': ev0.elapsed_time(ev1) / reps, 'calls': reps, 'shape': what}
+        log('%-40s %8.3f ms  (%s)' % (name, out[name]['ms_per_call'], what))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -182,6 +265,8 @@ def main():
     ap.add_argument('--traffic-probe', action='store_true',
                     help='only run 3 + 3 finest K*u launches (FP64, FP32) and the 2 GiB copies that calibrate '
                          'the FETCH_SIZE / WRITE_SIZE counters; for rocprofv3 --pmc runs')
+    ap.add_argument('--mex', type=int, default=0, metavar='REPS',
+                    help='only time the six MEX entry points (REPS calls each) at the --size shapes')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: the hot path has no multi-GPU mode in this build '
@@ -189,6 +274,9 @@ def main():
         sys.exit(2)
     if args.traffic_probe:
         traffic_probe(tuple(args.size))
+        return
+    if args.mex:
+        print(json.dumps({'mex_ms_per_call': mex_timing(tuple(args.size), args.mex)}), flush=True)
         return
     import torch
     shape = tuple(args.size)

@@ -144,3 +144,42 @@ def test_entries_accept_empty_inputs():
     mex.assembleKs_level2_superEle_inplace(empty_ks, _ga(np.zeros((0, 64), np.int32)), _ga(ms.I_mat @ Esub),
                                            _ga(Pch), _ga(ms.mapUniqueKes), _ga(Esub), _ga(ms.uniqueKesFree),
                                            _ga(ms.uniqueKesFixed))
+
+
+def test_scatter_incidence_cache_follows_the_connectivity():
+    """The scatter entry points keep the sorted incidence per connectivity; reuse must follow the
+    array's content (a connectivity rewritten in place is sorted again) and stay bitwise repeatable."""
+    import torch
+    from top_gpu_on_matlab_amd import mex
+    rng = np.random.default_rng(3)
+    Nn, Ne = 3000, 2000
+    eA = rng.integers(1, Nn + 1, size=(Ne, 8)).astype(np.int32)
+    eB = rng.integers(1, Nn + 1, size=(Ne, 8)).astype(np.int32)
+    V = rng.standard_normal((Ne, 24))
+    gV = _ga(V)
+    gA, gB = _ga(eA), _ga(eB)
+
+    def run(gE):
+        gY = _ga(np.zeros((Nn, 3)))
+        mex.Scattering_inplace(gY, gE, gV)
+        return _np(gY)
+
+    rA, rB = S.scattering(np.zeros((Nn, 3)), eA, V), S.scattering(np.zeros((Nn, 3)), eB, V)
+    yA = run(gA)
+    assert np.abs(yA - rA).max() < 1e-13 * np.abs(rA).max()
+    assert np.array_equal(run(gA), yA)                              # cached: identical bits
+    assert np.abs(run(gB) - rB).max() < 1e-13 * np.abs(rB).max()    # a second connectivity
+    assert np.array_equal(run(gA), yA)
+    gA.t.copy_(torch.from_numpy(np.ascontiguousarray(eB.reshape(-1, order='F'))).cuda())  # rewritten in place
+    assert np.abs(run(gA) - rB).max() < 1e-13 * np.abs(rB).max()
+    # scatter_accum3: the same keys array rewritten in place
+    keys = rng.integers(1, 400, size=5000).astype(np.int32)
+    Tm = rng.standard_normal((5000, 3))
+    gk, gT = _ga(keys), _ga(Tm)
+    for it in range(2):
+        gX = _ga(np.zeros((390, 3)))
+        mex.scatter_accum3_inplace(gk, gT, gX)
+        ref = S.scatter_accum3(keys, Tm, np.zeros((390, 3)))
+        assert np.abs(_np(gX) - ref).max() < 1e-13 * np.abs(ref).max()
+        keys = rng.integers(1, 400, size=5000).astype(np.int32)
+        gk.t.copy_(torch.from_numpy(keys).cuda())

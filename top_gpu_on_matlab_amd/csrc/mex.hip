@@ -3,48 +3,18 @@
 // Argument order, in-place contracts, class/shape checks and error identifiers follow the
 // reference gateways; the kernels are this library's own deterministic gfx950 kernels:
 //   * gather: one lane per element, coalesced column writes;
-//   * scatter / scatter_accum3: a stable radix sort of (target row, entry) pairs, then one lane
-//     per target row sums its entries in ascending entry order (no floating-point atomics);
+//   * scatter / scatter_accum3: a stable radix sort of (target row, entry) pairs, kept per
+//     connectivity as a CSR incidence, then one lane per target row sums its entries in ascending
+//     entry order (no floating-point atomics);
 //   * Galerkin assembly: one wavefront per coarse element (galerkin.h).
-#include <hipcub/hipcub.hpp>
+#include <memory>
+#include <mutex>
 
 #include "galerkin.h"
-#include "tg_internal.h"
+#include "scatter.h"
 
 namespace tg {
 namespace {
-
-struct Arr {
-    const tg_array *a = nullptr;
-    int64_t d[4] = {1, 1, 1, 1};
-    int nd = 0;  // number of dims after dropping trailing singletons beyond 2 (MATLAB ndims)
-    int64_t numel = 0;
-    int cls = TG_CLASS_NONE;
-    void *p = nullptr;
-};
-
-Arr arr(const tg_array *a) {
-    Arr r;
-    r.a = a;
-    if (!a) return r;
-    r.cls = a->cls;
-    r.p = a->data;
-    int nd = a->ndim < 0 ? 0 : (a->ndim > 4 ? 4 : a->ndim);
-    r.numel = 1;
-    for (int i = 0; i < nd; ++i) {
-        r.d[i] = a->dims[i];
-        r.numel *= a->dims[i];
-    }
-    if (nd == 0) r.numel = 0;
-    int md = nd;
-    while (md > 2 && r.d[md - 1] == 1) --md;
-    r.nd = md < 2 ? 2 : md;
-    return r;
-}
-
-bool is_float(int c) { return c == TG_SINGLE || c == TG_DOUBLE; }
-
-hipStream_t S(void *s) { return (hipStream_t)s; }
 
 // ---- gather ----------------------------------------------------------------------------
 template <typename T, typename IDX>
@@ -64,90 +34,167 @@ __global__ void k_gather(T *__restrict__ uMat, const T *__restrict__ u, const ID
         }
 }
 
-// ---- deterministic keyed accumulation ---------------------------------------------------
-// keys (0-based target rows; out-of-range rows mapped to M) are sorted stably with their entry
-// ids; row r then owns the contiguous run of entries whose key is r.
-__global__ void k_make_keys_enod(const int32_t *__restrict__ eNod, int64_t Ne, int64_t M, uint32_t *keys,
-                                 uint32_t *vals) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // i = 8e + j (element-major)
-    if (i >= Ne * 8) return;
-    const int64_t e = i >> 3, j = i & 7;
-    const int64_t k = (int64_t)eNod[e + Ne * j] - 1;
-    keys[i] = (k >= 0 && k < M) ? (uint32_t)k : (uint32_t)M;
-    vals[i] = (uint32_t)i;
+// Row pointers of a sorted key array: rowptr[r] = first position with key >= r, r = 0..M.
+__global__ void k_rowptr(const uint32_t *__restrict__ skeys, int64_t N, int64_t M, uint32_t *__restrict__ rowptr) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > M) return;
+    rowptr[r] = (uint32_t)lower_bound_u32(skeys, N, (uint32_t)r);
 }
 
-__global__ void k_make_keys_list(const int32_t *__restrict__ keysIn, int64_t N, int64_t M, uint32_t *keys,
-                                 uint32_t *vals) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    const int64_t k = (int64_t)keysIn[i] - 1;
-    keys[i] = (k >= 0 && k < M) ? (uint32_t)k : (uint32_t)M;
-    vals[i] = (uint32_t)i;
-}
-
-__device__ inline int64_t lower_bound_u32(const uint32_t *a, int64_t n, uint32_t v) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (a[mid] < v) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Y(r, c) += sum over sorted entries with key r of uMat(e, c + 3j), entry = 8e + j
+// Y(r, c) += sum of uMat(e, c + 3j) over the entries 8e + j of row r (ascending), CSR form.
 template <typename T>
-__global__ void k_seg_scatter_enod(T *__restrict__ Y, int64_t M, const uint32_t *__restrict__ skeys,
-                                   const uint32_t *__restrict__ svals, int64_t N, const T *__restrict__ uMat,
-                                   int64_t Ne) {
+__global__ void k_csr_scatter_enod(T *__restrict__ Y, int64_t M, const uint32_t *__restrict__ rowptr,
+                                   const uint32_t *__restrict__ ent, const T *__restrict__ uMat, int64_t Ne) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= M) return;
-    int64_t lo = lower_bound_u32(skeys, N, (uint32_t)r), hi = lower_bound_u32(skeys, N, (uint32_t)r + 1);
+    const uint32_t lo = rowptr[r], hi = rowptr[r + 1];
+    if (hi == lo) return;
     T a0 = 0, a1 = 0, a2 = 0;
-    for (int64_t k = lo; k < hi; ++k) {
-        const int64_t ent = svals[k], e = ent >> 3, j = ent & 7;
+    for (uint32_t k = lo; k < hi; ++k) {
+        const int64_t en = ent[k], e = en >> 3, j = en & 7;
         a0 += uMat[e + Ne * (0 + 3 * j)];
         a1 += uMat[e + Ne * (1 + 3 * j)];
         a2 += uMat[e + Ne * (2 + 3 * j)];
     }
-    if (hi > lo) {
-        Y[r] += a0;
-        Y[r + M] += a1;
-        Y[r + 2 * M] += a2;
-    }
+    Y[r] += a0;
+    Y[r + M] += a1;
+    Y[r + 2 * M] += a2;
 }
 
+// xF(r, c) += sum of Tmp(i, c) over the entries i of row r (ascending), CSR form.
 template <typename T>
-__global__ void k_seg_scatter_list(T *__restrict__ xF, int64_t M, const uint32_t *__restrict__ skeys,
-                                   const uint32_t *__restrict__ svals, int64_t N, const T *__restrict__ Tmp) {
+__global__ void k_csr_scatter_list(T *__restrict__ xF, int64_t M, const uint32_t *__restrict__ rowptr,
+                                   const uint32_t *__restrict__ ent, int64_t N, const T *__restrict__ Tmp) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= M) return;
-    int64_t lo = lower_bound_u32(skeys, N, (uint32_t)r), hi = lower_bound_u32(skeys, N, (uint32_t)r + 1);
+    const uint32_t lo = rowptr[r], hi = rowptr[r + 1];
+    if (hi == lo) return;
     T a0 = 0, a1 = 0, a2 = 0;
-    for (int64_t k = lo; k < hi; ++k) {
-        const int64_t i = svals[k];
+    for (uint32_t k = lo; k < hi; ++k) {
+        const int64_t i = ent[k];
         a0 += Tmp[i];
         a1 += Tmp[i + N];
         a2 += Tmp[i + 2 * N];
     }
-    if (hi > lo) {
-        xF[r] += a0;
-        xF[r + M] += a1;
-        xF[r + 2 * M] += a2;
-    }
+    xF[r] += a0;
+    xF[r + M] += a1;
+    xF[r + 2 * M] += a2;
 }
 
-// Sort (key, entry) pairs stably; returns sorted arrays in the provided buffers.
-void sort_pairs(uint32_t *keys, uint32_t *vals, uint32_t *skeys, uint32_t *svals, int64_t n, int64_t M,
-                hipStream_t st, const char *id) {
-    int bits = 1;
-    while ((1ULL << bits) <= (unsigned long long)M) ++bits;
-    size_t tmp = 0;
-    TG_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, skeys, vals, svals, (int)n, 0, bits, st), id);
-    DevBuf<char> t(tmp ? tmp : 1);
-    TG_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, tmp, keys, skeys, vals, svals, (int)n, 0, bits, st), id);
-    TG_HIP(hipStreamSynchronize(st), id);  // t is freed on return
+// Content hash of an int32 array: sum over i of mix(i) * (a_i + 2^32 n) mod 2^64 (integer adds, so the
+// result does not depend on the order in which the blocks arrive).
+__device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_hash_i32(const int32_t *__restrict__ a, int64_t n,
+                                                  unsigned long long *__restrict__ out) {
+    __shared__ unsigned long long part[256];
+    unsigned long long h = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        h += mix64((uint64_t)i) * ((uint64_t)(uint32_t)a[i] + ((uint64_t)n << 32));
+    part[threadIdx.x] = h;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(out, part[0]);
+}
+
+// ---- cached node-owned incidence ------------------------------------------------------------
+// The reference's scatter entry points are called with the same connectivity many times (every K*u,
+// restriction and prolongation of TOP_GPU.m).  The (row, entry) incidence of a connectivity is
+// sorted once and kept as CSR; a later call reuses it when the device, pointer, dims and a content
+// hash of the index array all match (the hash is recomputed on every call, so an array rewritten in
+// place is sorted again).
+struct Incidence {
+    int dev = -1;
+    const void *src = nullptr;
+    int kind = 0;  // 0 = eNodMat (entries 8e + j), 1 = key list (entries i)
+    int64_t n = 0, m = 0;
+    unsigned long long hash = 0;
+    uint64_t used = 0;
+    DevBuf<uint32_t> rowptr, ent;
+};
+
+struct IncidenceCache {
+    std::mutex mu;
+    std::vector<std::unique_ptr<Incidence>> items;
+    uint64_t tick = 0;
+    DevBuf<unsigned long long> hbuf[64];  // per-device hash accumulator
+};
+
+IncidenceCache &inc_cache() {
+    static IncidenceCache c;
+    return c;
+}
+
+constexpr size_t INC_CACHE_MAX = 8;
+
+// Returns the incidence for `idx` (eNodMat [Ne x 8] when kind == 0, keys [N] when kind == 1) with
+// rows 0..M-1; entries of out-of-range ids are left out.
+const Incidence &incidence(const int32_t *idx, int kind, int64_t Ne_or_N, int64_t M, hipStream_t st, const char *id) {
+    const int64_t N = kind == 0 ? 8 * Ne_or_N : Ne_or_N;
+    check_items(N, id);
+    int dev = 0;
+    TG_HIP(hipGetDevice(&dev), id);
+    if (dev < 0 || dev >= 64) fail(TG_ERR_HIP, id, "device ordinal %d out of range", dev);
+    IncidenceCache &C = inc_cache();
+    std::lock_guard<std::mutex> lock(C.mu);
+    if (!C.hbuf[dev].p) C.hbuf[dev].alloc(1);
+    TG_HIP(hipMemsetAsync(C.hbuf[dev].p, 0, sizeof(unsigned long long), st), id);
+    k_hash_i32<<<(unsigned)std::min<int64_t>(std::max<int64_t>((N + 255) / 256, 1), 2048), 256, 0, st>>>(idx, N, C.hbuf[dev].p);
+    TG_LAUNCH_CHECK(id);
+    unsigned long long h = 0;
+    TG_HIP(hipMemcpyAsync(&h, C.hbuf[dev].p, sizeof(h), hipMemcpyDeviceToHost, st), id);
+    TG_HIP(hipStreamSynchronize(st), id);
+    for (auto &it : C.items)
+        if (it->dev == dev && it->src == idx && it->kind == kind && it->n == N && it->m == M && it->hash == h) {
+            it->used = ++C.tick;
+            return *it;
+        }
+    // build: keys and entry ids, stable sort, row pointers
+    std::unique_ptr<Incidence> inc(new Incidence);
+    {
+        DevBuf<uint32_t> k0(N), k1(N), v0(N), v1(N);
+        if (kind == 0) k_make_keys_enod<<<grid_for(N, 256), 256, 0, st>>>(idx, Ne_or_N, M, k0, v0);
+        else k_make_keys_list<<<grid_for(N, 256), 256, 0, st>>>(idx, N, M, k0, v0);
+        TG_LAUNCH_CHECK(id);
+        hipcub::DoubleBuffer<uint32_t> kb(k0.p, k1.p), vb(v0.p, v1.p);
+        size_t tmp = 0;
+        sort_pairs_db(kb, vb, N, M, nullptr, tmp, st, id);
+        DevBuf<char> t(tmp ? tmp : 1);
+        sort_pairs_db(kb, vb, N, M, t.p, tmp, st, id);
+        inc->rowptr.alloc(M + 1);
+        k_rowptr<<<grid_for(M + 1, 256), 256, 0, st>>>(kb.Current(), N, M, inc->rowptr);
+        TG_LAUNCH_CHECK(id);
+        inc->ent.alloc(N);
+        TG_HIP(hipMemcpyAsync(inc->ent.p, vb.Current(), N * sizeof(uint32_t), hipMemcpyDeviceToDevice, st), id);
+        TG_HIP(hipStreamSynchronize(st), id);  // the temporaries are freed on leaving the scope
+    }
+    inc->dev = dev;
+    inc->src = idx;
+    inc->kind = kind;
+    inc->n = N;
+    inc->m = M;
+    inc->hash = h;
+    inc->used = ++C.tick;
+    // replace a stale entry of the same array, else the least recently used one when full
+    size_t slot = C.items.size();
+    for (size_t i = 0; i < C.items.size(); ++i)
+        if (C.items[i]->dev == dev && C.items[i]->src == idx && C.items[i]->kind == kind) slot = i;
+    if (slot == C.items.size() && C.items.size() >= INC_CACHE_MAX) {
+        slot = 0;
+        for (size_t i = 1; i < C.items.size(); ++i)
+            if (C.items[i]->used < C.items[slot]->used) slot = i;
+    }
+    if (slot == C.items.size()) C.items.push_back(std::move(inc));
+    else C.items[slot] = std::move(inc);
+    return *C.items[slot];
 }
 
 bool is_device_ptr(const void *p) {
@@ -281,27 +328,17 @@ int Gathering_inplace(int nlhs, tg_array *, int nrhs, const tg_array *prhs, void
 
 int Scattering_inplace(int, tg_array *, int nrhs, const tg_array *prhs, void *stream) {
     return guard("Scattering_inplace:arity", [&] {
-        const char *Sh = "Scattering_inplace:shape", *Ty = "Scattering_inplace:type", *Cu = "Scattering_inplace:cuda";
-        if (nrhs != 3 || !prhs) fail(TG_ERR_ARGS, "Scattering_inplace:arity", "Expected 3 inputs: (Y, eNodMat, uMat). No outputs.");
-        Arr Y = arr(&prhs[0]), en = arr(&prhs[1]), um = arr(&prhs[2]);
-        if (Y.nd != 2 || Y.d[1] != 3) fail(TG_ERR_SHAPE, Sh, "Y must be [numNodes x 3].");
-        if (!is_float(Y.cls)) fail(TG_ERR_TYPE, Ty, "Y must be single or double gpuArray.");
-        if (en.cls != TG_INT32) fail(TG_ERR_TYPE, Ty, "eNodMat must be gpuArray(int32).");
-        if (en.nd != 2 || en.d[1] != 8) fail(TG_ERR_SHAPE, Sh, "eNodMat must be [Ne x 8] int32 (1-based).");
-        const int64_t Ne = en.d[0], Nn = Y.d[0];
-        if (um.nd != 2 || um.d[0] != Ne || um.d[1] != 24) fail(TG_ERR_SHAPE, Sh, "uMat must be [Ne x 24] double.");
-        if (um.cls != Y.cls) fail(TG_ERR_TYPE, Ty, "uMat must have the class of Y.");
+        const char *Cu = "Scattering_inplace:cuda";
+        const ScatterArgs a = scatter_args(nrhs, prhs);
         hipStream_t st = S(stream);
-        if (Ne > 0 && Nn > 0) {
-            const int64_t N = Ne * 8;
-            DevBuf<uint32_t> k(N), v(N), sk(N), sv(N);
-            k_make_keys_enod<<<grid_for(N, 256), 256, 0, st>>>((const int32_t *)en.p, Ne, Nn, k, v);
-            TG_LAUNCH_CHECK(Cu);
-            sort_pairs(k, v, sk, sv, N, Nn, st, Cu);
-            if (Y.cls == TG_DOUBLE)
-                k_seg_scatter_enod<double><<<grid_for(Nn, 256), 256, 0, st>>>((double *)Y.p, Nn, sk, sv, N, (const double *)um.p, Ne);
+        if (a.Ne > 0 && a.Nn > 0) {
+            const Incidence &inc = incidence((const int32_t *)a.en.p, 0, a.Ne, a.Nn, st, Cu);
+            if (a.Y.cls == TG_DOUBLE)
+                k_csr_scatter_enod<double><<<grid_for(a.Nn, 256), 256, 0, st>>>((double *)a.Y.p, a.Nn, inc.rowptr, inc.ent,
+                                                                               (const double *)a.um.p, a.Ne);
             else
-                k_seg_scatter_enod<float><<<grid_for(Nn, 256), 256, 0, st>>>((float *)Y.p, Nn, sk, sv, N, (const float *)um.p, Ne);
+                k_csr_scatter_enod<float><<<grid_for(a.Nn, 256), 256, 0, st>>>((float *)a.Y.p, a.Nn, inc.rowptr, inc.ent,
+                                                                              (const float *)a.um.p, a.Ne);
             TG_LAUNCH_CHECK(Cu);
         }
         TG_HIP(hipStreamSynchronize(st), Cu);
@@ -327,18 +364,18 @@ int scatter_accum3_inplace(int nlhs, tg_array *, int nrhs, const tg_array *prhs,
         if (tmp.cls != xf.cls) fail(TG_ERR_TYPE, Ty, "`Tmp` and `xFiner` must have the same class (single/double).");
         if (!is_float(tmp.cls)) fail(TG_ERR_TYPE, Ty, "Only single or double supported for `Tmp`/`xFiner`.");
         if (keys.numel != N) fail(TG_ERR_SHAPE, Sh, "`transferMat` must hold exactly N = size(Tmp,1) keys.");
+        check_items(N, Sh);
         if (N == 0 || M == 0) return;
         hipStream_t st = S(stream);
-        DevBuf<uint32_t> k(N), v(N), sk(N), sv(N);
-        k_make_keys_list<<<grid_for(N, 256), 256, 0, st>>>((const int32_t *)keys.p, N, M, k, v);
-        TG_LAUNCH_CHECK(Cu);
-        sort_pairs(k, v, sk, sv, N, M, st, Cu);
+        const Incidence &inc = incidence((const int32_t *)keys.p, 1, N, M, st, Cu);
         if (xf.cls == TG_DOUBLE)
-            k_seg_scatter_list<double><<<grid_for(M, 256), 256, 0, st>>>((double *)xf.p, M, sk, sv, N, (const double *)tmp.p);
+            k_csr_scatter_list<double><<<grid_for(M, 256), 256, 0, st>>>((double *)xf.p, M, inc.rowptr, inc.ent, N,
+                                                                         (const double *)tmp.p);
         else
-            k_seg_scatter_list<float><<<grid_for(M, 256), 256, 0, st>>>((float *)xf.p, M, sk, sv, N, (const float *)tmp.p);
+            k_csr_scatter_list<float><<<grid_for(M, 256), 256, 0, st>>>((float *)xf.p, M, inc.rowptr, inc.ent, N,
+                                                                        (const float *)tmp.p);
         TG_LAUNCH_CHECK(Cu);
-        TG_HIP(hipStreamSynchronize(st), Cu);  // scratch buffers are released on return
+        TG_HIP(hipStreamSynchronize(st), Cu);
     });
 }
 

@@ -108,32 +108,52 @@ class Run:
 
 
 def cpu_baseline(shape, E, F, its_ref, sample_iters):
-    """Oracle (numpy/scipy restatement of the reference algorithm) on a bounded sample of the same
-    system: setup + Galerkin assembly, the first V-cycle, then `sample_iters` PCG iterations; the
-    solve time is extrapolated with the GPU FP64 solve's mean iteration count."""
+    """CPU restatement of the reference algorithm (oracle/, numpy/scipy, FP64), on one core: BLAS is held
+    to one thread and the oracle's sparse products and indexing are single-threaded anyway.
+
+    Two samples, about 20-30 s of CPU work in all:
+      * the benchmarked system itself (same shape, the first timed iteration's moduli): set-up and
+        Galerkin assembly (reported, not part of solves/s), the first V-cycle and `sample_iters` PCG
+        iterations; solves/s = 1 / (V-cycle + its_ref iterations), its_ref being the GPU FP64 solver's
+        mean iteration count on the same systems;
+      * a complete warm-start-free PCG solve of the 24 x 48 x 24 cantilever at its first optimisation
+        iteration (uniform density 0.12, ft = 1), timed whole: no extrapolation.
+    """
+    from threadpoolctl import threadpool_limits
     from oracle import mesh as M
     from oracle import solver as S
-    t0 = time.perf_counter()
-    m = M.build_cuboid_model(*shape, opt_case=1)
-    sv = S.Solver(m)
-    sv.assemble(E)
-    t1 = time.perf_counter()
-    log('cpu oracle setup+assembly %.1fs' % (t1 - t0))
-    ta = time.perf_counter()
-    sv.vcycle(F.copy())
-    tb = time.perf_counter()
-    S.pcg(sv.kbyu, sv.vcycle, F, 1e-30, sample_iters)
-    tc = time.perf_counter()
-    t_iter = ((tc - tb) - (tb - ta)) / sample_iters
-    t_solve = (tb - ta) + its_ref * t_iter
-    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        m = M.build_cuboid_model(*shape, opt_case=1)
+        sv = S.Solver(m)
+        sv.assemble(E)
+        t1 = time.perf_counter()
+        log('cpu oracle setup+assembly %.1fs' % (t1 - t0))
+        ta = time.perf_counter()
+        sv.vcycle(F.copy())
+        tb = time.perf_counter()
+        S.pcg(sv.kbyu, sv.vcycle, F, 1e-30, sample_iters)
+        tc = time.perf_counter()
+        t_iter = ((tc - tb) - (tb - ta)) / sample_iters
+        t_solve = (tb - ta) + its_ref * t_iter
+        log('cpu oracle: V-cycle %.1fs, %.2f s per PCG iteration' % (tb - ta, t_iter))
+        # complete solve of the small cantilever
+        mc = M.build_cuboid_model(24, 48, 24, opt_case=1)
+        svc = S.Solver(mc)
+        Ec = np.full(mc.levels[0].numElements, 1e-6 + 0.12 ** 3 * (1 - 1e-6))
+        svc.assemble(Ec)
+        td = time.perf_counter()
+        _, its_c, _ = svc.pcg(mc.F[:, 0], tol=1e-3, maxIT=800)
+        te = time.perf_counter()
+        log('cpu oracle: 24x48x24 solve %.2fs (%d PCG iterations)' % (te - td, its_c))
     return {
-        'value': 1.0 / t_solve, 'unit': 'solves/s', 'cores': threads, 'kind': 'port',
-        'sample': ('oracle/ (numpy/scipy restatement): setup + assembly on the first timed iteration\'s '
-                   'densities, first V-cycle + %d PCG iterations of that %dx%dx%d solve; solve time = '
-                   'V-cycle + %.1f iterations (GPU FP64 mean) x %.2f s/iteration; BLAS threaded, '
-                   'indexing single-threaded' % (sample_iters, shape[0], shape[1], shape[2], its_ref, t_iter)),
+        'value': 1.0 / t_solve, 'unit': 'solves/s', 'cores': 1, 'kind': 'port',
+        'sample': ('CPU restatement of the reference algorithm (oracle/, numpy/scipy, FP64, one thread): the '
+                   '%dx%dx%d system of the first timed iteration, first V-cycle + %d PCG iterations timed, solve '
+                   '= V-cycle + %.1f iterations (the GPU FP64 mean) x %.2f s/iteration; set-up and assembly '
+                   '%.1f s not included' % (shape[0], shape[1], shape[2], sample_iters, its_ref, t_iter, t1 - t0)),
         'sec_per_pcg_iteration': t_iter, 'setup_s': t1 - t0,
+        'full_solve_24x48x24': {'solves_per_s': 1.0 / (te - td), 'pcg_its': its_c, 'seconds': te - td},
     }
 
 
@@ -261,10 +281,13 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--size', type=int, nargs=3, default=[128, 256, 128], metavar=('NELY', 'NELX', 'NELZ'))
     ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--cpu-iters', type=int, default=1)
+    ap.add_argument('--cpu-iters', type=int, default=3)
     ap.add_argument('--traffic-probe', action='store_true',
                     help='only run 3 + 3 finest K*u launches (FP64, FP32) and the 2 GiB copies that calibrate '
                          'the FETCH_SIZE / WRITE_SIZE counters; for rocprofv3 --pmc runs')
+    ap.add_argument('--kfine-walsh', action='store_true',
+                    help='mixed solver: FP32 finest element products in the Walsh form on the VALU instead of the '
+                         'matrix cores (for comparison)')
     ap.add_argument('--mex', type=int, default=0, metavar='REPS',
                     help='only time the six MEX entry points (REPS calls each) at the --size shapes')
     args = ap.parse_args()
@@ -285,6 +308,8 @@ def main():
     x0 = 0.12 + rng.uniform(-0.05, 0.05, Ne_full)
     t = time.perf_counter()
     run = Run(shape, x0)
+    if args.kfine_walsh:
+        run.solvers['mixed'].set_tuning(kfine_mfma=False)
     torch.cuda.synchronize()
     log('setup + solid solves %.1fs (Ne=%d)' % (time.perf_counter() - t, run.Ne))
     hist = []

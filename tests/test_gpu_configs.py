@@ -70,7 +70,7 @@ def test_value_noise_domain_history_against_golden(tmp_path, non_dyadic):
 
 def test_super_element_mbb_full_size_matches_oracle():
     """MBB beam, 10 x 60 x 10 voxels, super-element mode (4x4x4 sub-voxels), ft = 2 (the convolution
-    filter in this mode), V0 = 0.2, rMin = sqrt(3), 3 iterations."""
+    filter in this mode), V0 = 0.2, rMin = sqrt(3), 3 iterations against the oracle run live."""
     from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
     shape = (10, 60, 10)
     om = M.build_cuboid_model(*shape, opt_case=2, super_element=True)
@@ -82,9 +82,44 @@ def test_super_element_mbb_full_size_matches_oracle():
         assert _rel(got[k], ref[k]) < 1e-10, (k, got[k], ref[k])
 
 
+def test_super_element_mbb_fifty_iterations_against_golden():
+    """The full super-element MBB configuration: 10 x 60 x 10, ft = 2, V0 = 0.2, 50 iterations, against
+    the oracle's committed history (tests/golden/c2_history.json).  The history is chaotic with respect to
+    rounding: the oracle with its loads perturbed at the unit-roundoff level (ulpPerturbed) moves by
+    1e-11 at iteration 5 and 4e-11 at iteration 7, and the device, whose rounding differs in every
+    operation, follows the same growth (6e-13 at iteration 1, 1.6e-10 at iteration 7).  So: iteration 1
+    within 1e-10; every iteration within 20x the oracle's largest spread up to that iteration (or 1e-10);
+    identical PCG counts while that spread stays below 1e-10."""
+    from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
+    g = json.load(open(os.path.join(ROOT, 'tests', 'golden', 'c2_history.json')))
+    c = g['case']
+    got = TOP_GPU(np.ones(c['shape'], bool), optCase=c['optCase'], V0=c['V0'], rMin=c['rMin'], nLoop=c['nLoop'],
+                  ft=c['ft'], super_element=1)
+    n = len(g['cHist'])
+    assert len(got['cHist']) == n
+    assert abs(got['cSolid'] - g['cSolid']) < 1e-10 * g['cSolid']
+    up = g['ulpPerturbed']
+    for key in ('cHist', 'volHist', 'sharpHist'):
+        ref = np.array(g[key])
+        spread = np.abs(np.array(up[key]) - ref) / np.abs(ref)
+        dev = np.abs(np.array(got[key]) - ref) / np.abs(ref)
+        K = int(np.argmax(spread > 1e-10)) if (spread > 1e-10).any() else n
+        run = np.maximum.accumulate(spread)
+        print('%s: oracle ulp-spread %s; device %s' % (key, ['%.1e' % v for v in spread], ['%.1e' % v for v in dev]),
+              flush=True)
+        assert K >= 2
+        assert dev[0] < 1e-10, key
+        assert np.all(dev <= np.maximum(20 * run, 1e-10)), (key, dev, run)
+        if key == 'cHist':
+            assert got['lssIts'][:K] == g['lssIts'][:K]
+
+
 def test_mixed_precision_cantilever_96x192x96_against_fp64():
-    """96 x 192 x 96 cantilever, ft = 2, V0 = 0.12: every FP64-driven iteration's system is solved by
-    the mixed solver too; compliance within 1e-6 relative (measured ~1e-11) and the same PCG counts."""
+    """The mixed-precision configuration at full length: 96 x 192 x 96 cantilever, ft = 2, rMin = 3,
+    V0 = 0.12, 50 optimisation iterations driven by the FP64 solver; every iteration's system is solved
+    by the mixed solver too (FP32 V-cycle with its finest element products on the matrix cores, FP64 PCG),
+    warm-started from its own previous solution.  Per iteration: relative compliance difference within
+    1e-6 (measured ~1e-11) and the same PCG count, or one more or fewer."""
     import torch
     from top_gpu_on_matlab_amd import _lib
     from top_gpu_on_matlab_amd import top_gpu as G
@@ -100,8 +135,8 @@ def test_mixed_precision_cantilever_96x192x96_against_fp64():
     xPhys = x.clone()
     U = {p: None for p in sv}
     ones = torch.ones_like(x)
-    errs = []
-    for it in range(5):
+    errs, its_hist = [], []
+    for it in range(50):
         E = G.simp(xPhys, 1e-6, 1.0, 3.0)
         c, its = {}, {}
         for p, s in sv.items():
@@ -110,15 +145,17 @@ def test_mixed_precision_cantilever_96x192x96_against_fp64():
             U[p], its[p], _ = s.pcg(F, y0=U[p])
             c[p] = float(torch.dot(F, U[p]))
         errs.append(abs(c['mixed'] - c['fp64']) / c['fp64'])
-        assert its['mixed'] == its['fp64'], (it, its)
+        its_hist.append((its['fp64'], its['mixed']))
         ce = G.unit_compliance(sv['fp64'], U['fp64'])
         dc = filt(G.sensitivity(xPhys, ce, 1.0, 1e-6, 1.0, 3.0))
         dv = filt(ones)
         fval = G.reduce(_lib.TG_RED_SUM, xPhys) / Ne - 0.12
         x, _ = G.oc_update(x, dc, dv, fval, 0.2)
         xPhys = filt(x)
-    print('\n[96x192x96 mixed vs fp64] rel compliance differences %s' % ['%.1e' % e for e in errs], flush=True)
+    print('\n[96x192x96 mixed vs fp64, 50 iterations] rel compliance difference per iteration %s; '
+          'PCG its (fp64, mixed) %s' % (['%.1e' % e for e in errs], its_hist), flush=True)
     assert max(errs) < 1e-6
+    assert all(abs(a - b) <= 1 for a, b in its_hist), its_hist
 
 
 @pytest.mark.parametrize('non_dyadic', [1, 0])
@@ -173,10 +210,11 @@ def test_value_noise_pcg_with_smaller_jacobi_weight_matches_oracle(tmp_path, non
 
 
 def test_value_noise_full_size_256x128x128_converges():
-    """The full 256 x 128 x 128 value-noise workload (1,676,295 voxels, 5.67M DOFs, .TopVoxel file
-    branch): with the reference's weight 0.6 every PCG solve runs to maxIT = 800 at a relative
-    residual near 0.1; with weightFactorJacobi = 0.4 three optimisation iterations converge
-    (< 300 PCG iterations each), keep the volume at V0 and lower the compliance."""
+    """The full 256 x 128 x 128 value-noise configuration (1,676,295 voxels, 5.67M DOFs, .TopVoxel file
+    branch, V0 = 0.4, ft = 2, rMin = 2) for its specified 30 iterations.  With the reference's weight
+    0.6 every PCG solve runs to maxIT = 800 at a relative residual near 0.1 (0.6 exceeds
+    2 / lambda_max(D^-1 K) on this domain); with weightFactorJacobi = 0.4 every solve converges (< 300 PCG
+    iterations), the volume stays at V0 and the compliance decreases overall."""
     import tempfile
     from top_gpu_on_matlab_amd.synthetic import bottom_clamp_top_patch, value_noise_mask
     from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
@@ -188,8 +226,11 @@ def test_value_noise_full_size_256x128x128_converges():
         path = os.path.join(tmp, 'value_noise.TopVoxel')
         write_topvoxel(path, mask, *bottom_clamp_top_patch(mask))
         gm = load_topvoxel_model(path)
-        got = TOP_GPU(path, V0=0.4, rMin=2.0, nLoop=3, ft=2, model=gm, weightFactorJacobi=0.4)
-    print('\n[value-noise 256x128x128, omega 0.4] its %s, c %s' % (got['lssIts'], got['cHist']), flush=True)
+        got = TOP_GPU(path, V0=0.4, rMin=2.0, nLoop=30, ft=2, model=gm, weightFactorJacobi=0.4)
+    print('\n[value-noise 256x128x128, omega 0.4, 30 iterations] its %s, c %s'
+          % (got['lssIts'], ['%.6e' % v for v in got['cHist']]), flush=True)
+    assert len(got['cHist']) == 30
     assert max(got['lssIts']) < 300
     assert all(abs(v - 0.4) < 1e-4 for v in got['volHist'])
     assert all(np.isfinite(got['cHist'])) and got['cHist'][2] < got['cHist'][1] < got['cHist'][0]
+    assert got['cHist'][-1] < got['cHist'][0]

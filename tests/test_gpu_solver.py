@@ -253,3 +253,41 @@ def test_mixed_vcycle_matches_fp64(super_element):
     assert float((z32 - z64).abs().max() / z64.abs().max()) < 2e-5
     a, b = float(torch.dot(sv['mixed'].vcycle(r1), r2)), float(torch.dot(r1, sv['mixed'].vcycle(r2)))
     assert abs(a - b) < 2e-5 * abs(a)
+
+
+@pytest.mark.parametrize('nu', [0.3, 0.27])
+def test_mixed_vcycle_matrix_core_element_products(nu):
+    """The FP32 finest K*u with its element products on the matrix cores (dense 24 x 24 on MFMA f32
+    32x32x2, the mixed V-cycle's default) against the Walsh form on the VALU, inside the mixed V-cycle,
+    on a perforated grid spanning several tiles and z chunks (chunk sizes 1, 3, 16): both differ from the
+    FP64 V-cycle by FP32 rounding only, and from each other by less."""
+    import torch
+    from top_gpu_on_matlab_amd.model import Model
+    from top_gpu_on_matlab_amd.solver import Solver
+    shape = (20, 33, 18)
+    vox = holey_mask(*shape, seed=7, frac=0.7)
+    sv = {}
+    for p in ('fp64', 'mixed'):
+        gm = Model(vox, poisson_ratio=nu, precision=p)
+        gm.builtin_case(1)
+        gm.finalize()
+        s = Solver(gm)
+        s.set_modulus(random_modulus(gm.num_elements, seed=5))
+        s.assemble()
+        sv[p] = s
+    r = torch.from_numpy(np.random.default_rng(8).standard_normal(sv['fp64'].ndof(1))).cuda()
+    z64 = sv['fp64'].vcycle(r)
+    sv['mixed'].set_tuning(kfine_mfma=False)
+    zw = sv['mixed'].vcycle(r)
+    sv['mixed'].set_tuning(kfine_mfma=True)
+    for zc in (1, 3, 16):
+        sv['mixed'].set_tuning(kfine_zchunk=zc)
+        zm = sv['mixed'].vcycle(r)
+        dm = float((zm - z64).abs().max() / z64.abs().max())
+        dw = float((zw - z64).abs().max() / z64.abs().max())
+        dmw = float((zm - zw).abs().max() / zw.abs().max())
+        print('nu %.2f chunk %d: mfma vs fp64 %.1e, walsh vs fp64 %.1e, mfma vs walsh %.1e' % (nu, zc, dm, dw, dmw),
+              flush=True)
+        assert dm < 2e-5 and dmw < 2e-5, (zc, dm, dw, dmw)
+        assert torch.equal(sv['mixed'].vcycle(r), zm)  # deterministic
+    torch.cuda.synchronize()

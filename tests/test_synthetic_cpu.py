@@ -97,3 +97,39 @@ def test_value_noise_domain_overdamps_jacobi_at_the_reference_weight(tmp_path):
     rq = float(v @ sv.kbyu(v, 1)) / float(v @ (dg * v))
     assert 0.6 * rq > 2.0 > 0.4 * rq, rq
     assert its[0.4] * 2 < its[0.6], its
+
+
+def test_unfiltered_random_density_overdamps_jacobi_at_the_reference_weight():
+    """Why the first optimisation iteration of the 128 x 256 x 128 benchmark runs both solvers to maxIT:
+    its densities are the unfiltered start x0 = 0.12 + U(-0.05, 0.05) (seeded), so neighbouring moduli
+    differ by up to 14x, and the largest eigenvalue of D^-1 K on the finest level rises from 3.09 (any
+    uniform field) to 3.9, past 2 / 0.6: the damped Jacobi smoother with the reference's weight amplifies
+    those modes.  The oracle's PCG on the same recipe at 16 x 32 x 16 needs several times the iterations
+    of omega = 0.5, which stays below 2 / lambda_max.  After the first density filter the field is smooth
+    and the weight 0.6 converges again."""
+    from oracle import solver as S
+    m = M.build_cuboid_model(16, 32, 16, opt_case=1)
+    Ne = m.levels[0].numElements
+    x0 = 0.12 + np.random.default_rng(12345).uniform(-0.05, 0.05, Ne)
+    E = 1e-6 + x0 ** 3 * (1 - 1e-6)
+
+    def lam_max(sv):
+        dg = sv.levels[0].diagK
+        v = np.random.default_rng(0).standard_normal(dg.size)
+        for _ in range(60):
+            w = sv.kbyu(v, 1) / dg
+            v = w / np.linalg.norm(w)
+        return float(v @ sv.kbyu(v, 1)) / float(v @ (dg * v))   # Rayleigh quotient: a lower bound
+
+    its = {}
+    for w in (0.6, 0.5):
+        sv = S.Solver(m, weight_factor_jacobi=w)
+        sv.assemble(E)
+        its[w] = sv.pcg(m.F[:, 0], 1e-3, 800)[1]
+    lam_rand = lam_max(sv)
+    su = S.Solver(m)
+    su.assemble(np.full(Ne, 1e-6 + 0.12 ** 3))
+    lam_unif = lam_max(su)
+    assert lam_unif * 0.6 < 2.0 < lam_rand * 0.6, (lam_unif, lam_rand)
+    assert lam_rand * 0.5 < 2.0
+    assert its[0.5] * 3 < its[0.6], its

@@ -53,7 +53,7 @@ def build(verbose=True, jobs=None):
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + '.tmp'
     cmd = [_hipcc(), '-shared', '-fPIC', '--offload-arch=' + ARCH, '-o', tmp] + objs + [
-        '-L' + os.path.join(ROCM, 'lib'), '-lrocblas', '-fopenmp',
+        '-L' + os.path.join(ROCM, 'lib'), '-fopenmp',
         '-Wl,-rpath,' + os.path.join(ROCM, 'lib')]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

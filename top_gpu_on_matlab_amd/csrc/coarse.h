@@ -1,20 +1,24 @@
 // Coarsest-level direct solver (replaces the host CHOLMOD factorisation and the per-V-cycle
 // host round trip of TOP_GPU.m:796-802, 1242-1256).
 //
-// The coarsest grid is cut into node planes along its longest axis, so K_c is block tridiagonal.
-// It is factorised from both ends towards a middle separator plane m ("twisted" block Cholesky):
-//   chain A, p = 0 .. m-1:     L_pp L_pp^T = A_pp - L_{p,p-1} L_{p,p-1}^T,  L_{p+1,p} = A_{p+1,p} L_pp^-T
-//   chain B, p = P-1 .. m+1:   L_pp L_pp^T = A_pp - L_{p,p+1} L_{p,p+1}^T,  L_{p-1,p} = A_{p-1,p} L_pp^-T
-//   separator:                 L_mm L_mm^T = A_mm - L_{m,m-1} L_{m,m-1}^T - L_{m,m+1} L_{m,m+1}^T
-// on the device (blocked Cholesky: LDS kernels for the diagonal blocks, panels and trailing updates,
-// the two chains' planes of a step sharing each launch; rocBLAS syrk/trsm/gemm for the couplings; all on
-// the solver stream).  A solve is a batched triangular product y = blockdiag(inv(L_pp)) b, the two
-// forward chains advanced together (one launch per step, ~P/2 steps) and the separator, then
-// x = blockdiag(inv(L_pp)^T) y and the two backward chains outwards from the separator.  The sequence
-// reads only buffers fixed at setup, so it can also be captured once as a HIP graph and replayed.
+// The coarsest grid is cut into node planes along its longest axis, so K_c is block tridiagonal with
+// dense diagonal blocks A_pp and sparse couplings A_{p,p+-1} (27 entries per row: the 3 x 3 node
+// neighbourhood in the next plane, 3 DOFs each).  It is eliminated from both ends towards a middle
+// separator plane m ("twisted" block LDL'), keeping the inverse of every plane's Schur complement:
+//   chain A, p = 0 .. m-1:    S_p = A_pp - A_{p,p-1} N_{p-1},     N_p = S_p^-1 A_{p,p+1}
+//   chain B, q = P-1 .. m+1:  S_q = A_qq - A_{q,q+1} N_{q+1},     N_q = S_q^-1 A_{q,q-1}
+//   separator:                S_m = A_mm - A_{m,m-1} N_{m-1} - A_{m,m+1} N_{m+1}
+// Each S_p^-1 is formed in place by a blocked Gauss-Jordan sweep (64-wide pivot blocks: the pivot
+// block's inverse in LDS, the column panel, then a rank-64 update of the whole plane block on the
+// matrix cores); the two chains' planes of a step share every launch.  The solve operators are
+//   forward  F_p = S_p^-1 A_{p,p-1} (chain A), S_q^-1 A_{q,q+1} (chain B), [S_m^-1 A_{m,m-1} | S_m^-1 A_{m,m+1}]
+//   backward B_p = N_p
+// and a solve is c = blockdiag(S_p^-1) b (all planes in one launch), the forward recurrences
+// z_p = c_p - F_p z_{p-1|p+1} of both chains advanced together (one launch per step), the separator
+// x_m = c_m - F_m [z_{m-1}; z_{m+1}], then x_p = z_p - B_p x_{p+1|p-1} outwards.  Everything is FP64
+// and deterministic (fixed summation orders, no atomics).  The solve sequence reads only buffers fixed
+// at setup, so it can also be captured once as a HIP graph and replayed.
 #pragma once
-
-#include <rocblas/rocblas.h>
 
 #include "kernels.h"
 #include "model.h"
@@ -25,23 +29,24 @@ struct CoarseSolver {
     int axis = 0;                   // 0 = y, 1 = x, 2 = z
     int P = 0;                      // number of planes
     std::vector<int64_t> nd;        // DOFs per plane
+    std::vector<int64_t> npad;      // DOFs per plane padded to a multiple of 64 (the sweep's blocks)
     std::vector<int64_t> off;       // DOF offset per plane
-    std::vector<int64_t> adOff, asOff;  // offsets of the diagonal / sub-diagonal plane blocks
+    std::vector<int64_t> sOff;      // offset of each plane's npad x npad block (row-major)
     int m = 0;                          // separator plane; chain A = [0, m), chain B = (m, P-1]
-    std::vector<int64_t> fOff, bOff;    // offsets of the padded forward / backward chain operators
+    std::vector<int64_t> fOff, bOff;    // offsets of the padded forward / backward operators
     std::vector<int64_t> fLd, bLd;      // their padded row lengths
     int64_t ntot = 0;
     Grid g{};
     DevBuf<int32_t> lidx;           // per full-grid node: local node index within its plane (-1 none)
-    DevBuf<int64_t> d_off, d_adOff, d_asOff;
-    int64_t npairs = 0;             // row pairs (r, n_p - 1 - r) over all planes
-    DevBuf<int32_t> pairPlane;      // plane of each row pair
-    DevBuf<int64_t> d_pairOff;      // first row pair of each plane
-    DevBuf<double> Ad, As, Linv, LT, Fop, Bop;
-    DevBuf<double> bb, yy, xx;
-    DevBuf<rocblas_int> info;
-    DevBuf<double> chW;             // inv(L_kk) of the current Cholesky panel
-    rocblas_handle h = nullptr;
+    DevBuf<int64_t> d_off, d_sOff, d_npad, d_nd;
+    DevBuf<int32_t> rowPlane;       // plane of each DOF row (solve-time block products)
+    DevBuf<double> Sinv;            // per plane: A_pp, then S_p, then S_p^-1 (row-major, padded)
+    DevBuf<double> dnV, upV;        // couplings to the previous / next plane, 27 per DOF row
+    DevBuf<int32_t> dnC, upC;       // their column (local DOF in that plane)
+    DevBuf<double> Fop, Bop;
+    DevBuf<double> T, Rw, Dk;       // sweep work: column panel, old pivot row, pivot inverse (2 planes)
+    DevBuf<double> bb, yy;
+    DevBuf<int> info;
     bool factored = false;
     hipGraphExec_t exec = nullptr;  // the captured solve sequence
     bool use_graph = false;         // replay the captured sequence (false: launch it directly)

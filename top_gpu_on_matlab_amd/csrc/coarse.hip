@@ -1,4 +1,4 @@
-// Coarsest-level direct solver: plane-blocked Cholesky on the device (see coarse.h).
+// Coarsest-level direct solver: twisted block-Schur-complement chain on the device (see coarse.h).
 #include "coarse.h"
 
 namespace tg {
@@ -6,14 +6,42 @@ namespace {
 
 __device__ inline int coord(int axis, int y, int x, int z) { return axis == 0 ? y : (axis == 1 ? x : z); }
 
-// One lane per (node J, neighbour slot, DOF pair): K_c(J c, M d) = sum over the solid elements
-// shared by J and M of Ks_e(3a_J + c, 3b_M + d)  (the entries sparse(iK, jK, sK) accumulates,
-// TOP_GPU.m:1249-1254).  Lower blocks only (plane(M) <= plane(J)).
+// In-plane neighbour index (0..8) of a node offset: the two offsets along the axes other than `axis`,
+// in (y, x, z) order.
+__device__ inline int inplane9(int axis, int dy, int dx, int dz) {
+    const int a = axis == 0 ? dx : dy, b = axis == 2 ? dx : dz;
+    return (a + 1) + 3 * (b + 1);
+}
+
+constexpr int CB = 64;  // pivot block of the Gauss-Jordan sweep (plane blocks are padded to multiples)
+constexpr int NCPL = 27;  // couplings per DOF row to one neighbouring plane (3 x 3 nodes x 3 DOFs)
+
+// Plane blocks before assembly: zero, with ones on the diagonal of the padding rows (n_p <= i < npad_p),
+// so the padded block is [A_pp 0; 0 I] and its inverse [S^-1 0; 0 I].
+__global__ void k_plane_init(double *__restrict__ S, const int64_t *__restrict__ sOff, const int64_t *__restrict__ npad,
+                             const int64_t *__restrict__ nd) {
+    const int p = blockIdx.y;
+    const int64_t np = npad[p], t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np * np) return;
+    const int64_t i = t / np, j = t - i * np;
+    S[sOff[p] + t] = (i == j && i >= nd[p]) ? 1.0 : 0.0;
+}
+
+// One lane per (node J, neighbour slot, DOF pair): K_c(J c, M d) = sum over the solid elements shared by
+// J and M of Ks_e(3a_J + c, 3b_M + d) (the entries sparse(iK, jK, sK) accumulates, TOP_GPU.m:1249-1254).
+// Same-plane entries go to the plane's dense block (row-major, padded); entries coupling J to the
+// previous / next plane go to the compact coupling rows (slot = in-plane neighbour x 3 + d, with the
+// neighbour's local DOF as column; absent neighbours get value 0 and column 0).  Only one triangle of
+// K_c is formed, as chol(K_c, 'lower') reads only one: entries above the diagonal in plane order are
+// summed from the transposed element entries Ks_e(3b + d, 3a + c), over the same elements in the same
+// (ascending) order as their mirror image, so the plane blocks are exactly symmetric and the coupling
+// to the next plane is exactly the transpose of the next plane's coupling to this one.
 __global__ void k_coarse_assemble(const double *__restrict__ Ks, int64_t ldk, const uint8_t *__restrict__ eleMask,
                                   const uint8_t *__restrict__ nodeMask, const Grid g, int axis,
                                   const int32_t *__restrict__ lidx, const int64_t *__restrict__ off,
-                                  const int64_t *__restrict__ adOff, const int64_t *__restrict__ asOff,
-                                  double *__restrict__ Ad, double *__restrict__ As) {
+                                  const int64_t *__restrict__ sOff, const int64_t *__restrict__ npad,
+                                  double *__restrict__ S, double *__restrict__ dnV, int32_t *__restrict__ dnC,
+                                  double *__restrict__ upV, int32_t *__restrict__ upC) {
     const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tid >= g.nodes() * 243) return;
     const int64_t J = tid / 243;
@@ -24,12 +52,19 @@ __global__ void k_coarse_assemble(const double *__restrict__ Ks, int64_t ldk, co
     const int64_t t = J / g.NY;
     const int jx = (int)(t % g.NX), jz = (int)(t / g.NX);
     const int dy = k % 3 - 1, dx = (k / 3) % 3 - 1, dz = k / 9 - 1;
+    const int pJ = coord(axis, jy, jx, jz), dp = coord(axis, dy, dx, dz);
+    const int64_t row = off[pJ] + 3 * (int64_t)lidx[J] + c;
+    const int slot = 3 * inplane9(axis, dy, dx, dz) + d;
     const int my = jy + dy, mx = jx + dx, mz = jz + dz;
-    if ((unsigned)my >= (unsigned)g.NY || (unsigned)mx >= (unsigned)g.NX || (unsigned)mz >= (unsigned)g.NZ) return;
-    const int64_t M = my + (int64_t)g.NY * (mx + (int64_t)g.NX * mz);
-    if (!(nodeMask[M] & NODE_ACTIVE)) return;
-    const int pJ = coord(axis, jy, jx, jz), pM = coord(axis, my, mx, mz);
-    if (pM > pJ) return;
+    const bool inside = (unsigned)my < (unsigned)g.NY && (unsigned)mx < (unsigned)g.NX && (unsigned)mz < (unsigned)g.NZ;
+    const int64_t M = inside ? my + (int64_t)g.NY * (mx + (int64_t)g.NX * mz) : 0;
+    if (!inside || !(nodeMask[M] & NODE_ACTIVE)) {
+        if (dp == -1) { dnV[row * NCPL + slot] = 0.0; dnC[row * NCPL + slot] = 0; }
+        if (dp == 1) { upV[row * NCPL + slot] = 0.0; upC[row * NCPL + slot] = 0; }
+        return;
+    }
+    const int64_t col = 3 * (int64_t)lidx[M] + d;
+    const bool tr = dp == 1 || (dp == 0 && col > row - off[pJ]);
     double val = 0;
     for (int oz = -1; oz <= 0; ++oz)
         for (int ox = -1; ox <= 0; ++ox)
@@ -42,302 +77,276 @@ __global__ void k_coarse_assemble(const double *__restrict__ Ks, int64_t ldk, co
                 const int64_t e = ey + (int64_t)g.EY * (ex + (int64_t)g.EX * ez);
                 if (!eleMask[e]) continue;
                 const int a = vtx_of(-oy, -ox, -oz), b = vtx_of(ty, tx, tz);
-                val += Ks[(int64_t)((3 * a + c) + 24 * (3 * b + d)) * ldk + e];
+                val += Ks[(int64_t)(tr ? (3 * b + d) + 24 * (3 * a + c) : (3 * a + c) + 24 * (3 * b + d)) * ldk + e];
             }
-    const int64_t np = off[pJ + 1] - off[pJ];
-    const int64_t row = 3 * (int64_t)lidx[J] + c, col = 3 * (int64_t)lidx[M] + d;
-    if (pM == pJ) Ad[adOff[pJ] + row + np * col] = val;
-    else As[asOff[pJ] + row + np * col] = val;
+    if (dp == 0) {
+        S[sOff[pJ] + (row - off[pJ]) * npad[pJ] + col] = val;
+    } else if (dp == -1) {
+        dnV[row * NCPL + slot] = val;
+        dnC[row * NCPL + slot] = (int32_t)col;
+    } else {
+        upV[row * NCPL + slot] = val;
+        upC[row * NCPL + slot] = (int32_t)col;
+    }
 }
 
-// ---- plane-block Cholesky: blocked right-looking, panels of CH_NB columns --------------------
-// Per panel: the kb x kb diagonal block is factorised in LDS by one workgroup, which also forms
-// W = inv(L_kk); the rows under it are X = P W^T (k_chol_panel); the trailing matrix is updated by
-// k_chol_update.  Column-major, lower triangle; the upper triangle is not referenced.
-constexpr int CH_NB = 64;
-constexpr int CH_PR = 32;  // panel rows per workgroup
+// ---- Schur complement update:  S(i, j) -= sum_k V(i, k) N(C(i, k), j)  (A_{p,p+-1} N, up to two terms) ----
+struct SchurTerm {
+    const double *V;
+    const int32_t *C;
+    const double *N;
+    int64_t ldN;
+};
+struct SchurProb {
+    double *S;
+    int64_t np, n;
+    int nt;
+    SchurTerm t[2];
+};
 
-// info: set to col0 + j + 1 (LAPACK convention, first failure only) when pivot j is not positive;
-// the factorisation then continues with a unit pivot so every value stays finite.
-// Steps of CH_RB columns: every lane factorises the CH_RB x CH_RB diagonal sub-block itself (from LDS,
-// in registers), the rows below are scaled (one lane per row), then a rank-CH_RB trailing update;
-// the inverse is formed in row blocks of CH_IB with one lane per (row in block, column).  Two barriers
-// per step and one per inverse row block.
-constexpr int CH_RB = 4;  // columns per factorisation step
-constexpr int CH_IB = 4;  // rows per inverse block (CH_IB lanes per column: 4 x 64 = 256)
-// Batched over blockIdx.y: problem y works on A + y * dA, W + y * CH_NB^2 and info[y * dinfo] (the two
-// elimination chains' planes of one step share a launch).
-__global__ __launch_bounds__(256) void k_chol_diag(double *__restrict__ A, int64_t lda, int kb,
-                                                   double *__restrict__ W, int *__restrict__ info, int col0,
-                                                   int64_t dA, int dinfo) {
-    __shared__ double a[CH_NB][CH_NB + 1];  // a[i][j] = A(i, j), factorised in place
-    __shared__ double w[CH_NB][CH_NB + 1];  // inv(L_kk)
-    __shared__ double rdg[CH_NB];           // 1 / L(j, j)
-    const int t = threadIdx.x;
-    A += blockIdx.y * dA;
-    W += blockIdx.y * (CH_NB * CH_NB);
-    info += blockIdx.y * dinfo;
-    for (int idx = t; idx < kb * kb; idx += 256) {
-        const int i = idx % kb, j = idx / kb;
-        a[i][j] = A[i + (int64_t)j * lda];
+__global__ __launch_bounds__(256) void k_schur(const SchurProb A, const SchurProb B) {
+    const SchurProb &q = blockIdx.z ? B : A;
+    const int64_t i = blockIdx.y, j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= q.n || j >= q.n) return;
+    double acc = 0.0;
+    for (int s = 0; s < q.nt; ++s) {
+        const double *V = q.t[s].V + i * NCPL;
+        const int32_t *C = q.t[s].C + i * NCPL;
+        const double *N = q.t[s].N + j;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < NCPL; k += 3) {
+            a0 += V[k] * N[(int64_t)C[k] * q.t[s].ldN];
+            a1 += V[k + 1] * N[(int64_t)C[k + 1] * q.t[s].ldN];
+            a2 += V[k + 2] * N[(int64_t)C[k + 2] * q.t[s].ldN];
+        }
+        acc += (a0 + a1) + a2;
     }
-    // rows/columns kb..CH_NB-1 of the padded block: identity, so every step can work on full blocks
-    for (int idx = t; idx < CH_NB * CH_NB; idx += 256) {
-        const int i = idx % CH_NB, j = idx / CH_NB;
-        if (i >= kb || j >= kb) a[i][j] = i == j ? 1.0 : 0.0;
-    }
-    __syncthreads();
+    q.S[i * q.np + j] -= acc;
+}
+
+// ---- blocked Gauss-Jordan sweep: S <- S^-1 in place, pivot blocks K = 0 .. nb-1 -------------------------
+// Per pivot block K (with D = S_KK^-1, computed from the current S):
+//   S_IJ -= S_IK D S_KJ (I, J != K),  S_IK <- S_IK D,  S_KJ <- -D S_KJ,  S_KK <- D,
+// which after the last block leaves the inverse (S is SPD, so every pivot block is the SPD Schur
+// complement of the blocks already swept and no pivoting is needed).  k_sweep_panel forms T_I = S_IK D
+// and copies the old pivot row Rw_I = S_KI; k_sweep_update then writes every block from T, Rw and D, so
+// no block is read after another workgroup has overwritten it.
+struct SweepProb {
+    double *S;      // npad x npad, row-major
+    int64_t np;     // npad
+    int nb;         // npad / CB
+    double *T;      // npad x CB   (T_I = S_IK D)
+    double *Rw;     // CB x npad   (old pivot row block)
+    double *D;      // CB x CB
+    int *info;      // first non-positive pivot (1-based, 0 = none)
+};
+
+// 64 x 64 SPD inverse by Gauss-Jordan in registers: thread t holds the 4 x 4 block at rows 4 (t / 16),
+// columns 4 (t % 16); each pivot's row and column pass through LDS (double-buffered, one barrier per
+// pivot).  Returns 1 + the first non-positive pivot index (0 if none; such a pivot is replaced by 1).
+__device__ int gj_inverse64(double (&r)[4][4], double (*rowb)[CB], double (*colb)[CB]) {
+    const int t = threadIdx.x, bi = 4 * (t / 16), bj = 4 * (t % 16);
     int bad = 0;
-    for (int j0 = 0; j0 < CH_NB; j0 += CH_RB) {
-        // the diagonal sub-block, factorised redundantly by every lane
-        // (one division per column; the rest of the step multiplies by the reciprocal pivot)
-        double L[CH_RB][CH_RB], rp[CH_RB];
+    for (int k = 0; k < CB; ++k) {
+        const int b = k & 1;
+        if (bi == (k & ~3)) {
 #pragma unroll
-        for (int r = 0; r < CH_RB; ++r)
-#pragma unroll
-            for (int c = 0; c <= r; ++c) L[r][c] = a[j0 + r][j0 + c];
-#pragma unroll
-        for (int c = 0; c < CH_RB; ++c) {
-            double d = L[c][c];
-#pragma unroll
-            for (int k = 0; k < c; ++k) d -= L[c][k] * L[c][k];
-            if (!(d > 0.0)) {
-                if (!bad) bad = j0 + c + 1;
-                d = 1.0;
-            }
-            L[c][c] = sqrt(d);
-            rp[c] = 1.0 / L[c][c];
-#pragma unroll
-            for (int r = c + 1; r < CH_RB; ++r) {
-                double v = L[r][c];
-#pragma unroll
-                for (int k = 0; k < c; ++k) v -= L[r][k] * L[c][k];
-                L[r][c] = v * rp[c];
-            }
+            for (int jj = 0; jj < 4; ++jj) rowb[b][bj + jj] = r[k & 3][jj];
         }
-        // rows below: x L^T = a(i, j0 .. j0+3)
-        const int i = j0 + CH_RB + t;
-        if (i < CH_NB) {
-            double x[CH_RB];
+        if (bj == (k & ~3)) {
 #pragma unroll
-            for (int c = 0; c < CH_RB; ++c) {
-                double v = a[i][j0 + c];
-#pragma unroll
-                for (int k = 0; k < c; ++k) v -= x[k] * L[c][k];
-                x[c] = v * rp[c];
-            }
-#pragma unroll
-            for (int c = 0; c < CH_RB; ++c) a[i][j0 + c] = x[c];
-        }
-        __syncthreads();  // every lane has read the sub-block; the scaled columns are complete
-        if (t == 0) {
-#pragma unroll
-            for (int r = 0; r < CH_RB; ++r) {
-#pragma unroll
-                for (int c = 0; c <= r; ++c) a[j0 + r][j0 + c] = L[r][c];
-                rdg[j0 + r] = rp[r];
-            }
-        }
-        // trailing update of the lower triangle past the step: lane (row ri, column phase cph)
-        const int ri = t % CH_NB, cph = t / CH_NB;
-        if (ri >= j0 + CH_RB) {
-            double ar[CH_RB];
-#pragma unroll
-            for (int c = 0; c < CH_RB; ++c) ar[c] = a[ri][j0 + c];
-            for (int l = j0 + CH_RB + cph; l <= ri; l += 4) {
-                double v = a[ri][l];
-#pragma unroll
-                for (int c = 0; c < CH_RB; ++c) v -= ar[c] * a[l][j0 + c];
-                a[ri][l] = v;
-            }
+            for (int ii = 0; ii < 4; ++ii) colb[b][bi + ii] = r[ii][k & 3];
         }
         __syncthreads();
-    }
-    // W = inv(L): row block i0 .. i0+3, lane (r = t % 4, column c = t / 4); the four lanes of a column
-    // are adjacent and pass the block's new rows along by shuffles
-    const int r = t % CH_IB, c = t / CH_IB;
-    for (int i0 = 0; i0 < CH_NB; i0 += CH_IB) {
-        const int i = i0 + r;
-        double sacc = 0.0, sacc2 = 0.0;  // sum over the finished rows k < i0 of L(i, k) W(k, c)
-        int k = c;
-        for (; k + 1 < i0; k += 2) {
-            sacc += a[i][k] * w[k][c];
-            sacc2 += a[i][k + 1] * w[k + 1][c];
+        double piv = colb[b][k];
+        if (!(piv > 0.0)) {
+            if (!bad) bad = k + 1;
+            piv = 1.0;
         }
-        if (k < i0) sacc += a[i][k] * w[k][c];
-        sacc += sacc2;
-        double wv = 0.0;
+        const double p = 1.0 / piv;
+        double ak[4], kb[4];
 #pragma unroll
-        for (int rr = 0; rr < CH_IB; ++rr) {
-            // row i0 + rr: lane rr finishes it with the block rows above it (held by lanes < rr)
-            double add = 0.0;
+        for (int ii = 0; ii < 4; ++ii) ak[ii] = colb[b][bi + ii];
 #pragma unroll
-            for (int q = 0; q < rr; ++q) {
-                const double wq = __shfl(wv, (t & ~(CH_IB - 1)) + q, 64);
-                if (r == rr) add += a[i][i0 + q] * wq;
+        for (int jj = 0; jj < 4; ++jj) kb[jj] = rowb[b][bj + jj];
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) {
+                const bool ri = bi + ii == k, cj = bj + jj == k;
+                const double v = r[ii][jj];
+                r[ii][jj] = ri ? (cj ? p : -p * kb[jj]) : (cj ? ak[ii] * p : v - (ak[ii] * p) * kb[jj]);
             }
-            if (r == rr) {
-                const double rii = rdg[i];
-                wv = c > i ? 0.0 : (c == i ? rii : -(sacc + add) * rii);
-            }
-        }
-        w[i][c] = wv;
-        __syncthreads();
     }
-    for (int idx = t; idx < kb * kb; idx += 256) {
-        const int ii = idx % kb, j = idx / kb;
-        if (ii >= j) A[ii + (int64_t)j * lda] = a[ii][j];
-        W[ii + CH_NB * j] = w[ii][j];
-    }
-    if (t == 0 && bad && bad <= kb && *info == 0) *info = col0 + bad;
+    return bad;
 }
 
-// Rows under the diagonal block: X = P W^T for the m x kb panel P (column-major, lda), in place;
-// CH_PR rows per workgroup, lane (row r, column group cg) forms 8 outputs.
-__global__ __launch_bounds__(256) void k_chol_panel(double *__restrict__ P, int64_t lda, int m, int kb,
-                                                    const double *__restrict__ W, int64_t dA) {
-    __shared__ double w[CH_NB][CH_NB + 1];
-    __shared__ double pr[CH_PR][CH_NB + 1];
-    const int t = threadIdx.x;
-    P += blockIdx.y * dA;  // batched like k_chol_diag
-    W += blockIdx.y * (CH_NB * CH_NB);
-    const int row0 = blockIdx.x * CH_PR;
-    const int rows = min(CH_PR, m - row0);
-    for (int idx = t; idx < kb * kb; idx += 256) {
-        const int i = idx % kb, j = idx / kb;
-        w[i][j] = W[i + CH_NB * j];
-    }
-    for (int idx = t; idx < rows * kb; idx += 256) {
-        const int r = idx % rows, l = idx / rows;
-        pr[r][l] = P[(row0 + r) + (int64_t)l * lda];
-    }
-    __syncthreads();
-    const int r = t % CH_PR, cg = t / CH_PR;  // 8 groups of 8 columns
-    if (r < rows) {
+__global__ __launch_bounds__(256) void k_sweep_panel(const SweepProb A, const SweepProb B, int K) {
+    __shared__ double dl[CB][CB + 1];   // D
+    __shared__ double sl[CB][CB + 1];   // S_IK
+    __shared__ double rowb[2][CB], colb[2][CB];
+    const SweepProb &q = blockIdx.y ? B : A;
+    const int I = blockIdx.x;
+    if (K >= q.nb || I >= q.nb) return;
+    const int t = threadIdx.x, bi = 4 * (t / 16), bj = 4 * (t % 16);
+    const int64_t np = q.np, k0 = (int64_t)K * CB, i0 = (int64_t)I * CB;
+    double r[4][4];
 #pragma unroll
-        for (int jj = 0; jj < CH_NB / 8; ++jj) {
-            const int j = cg * (CH_NB / 8) + jj;
-            if (j < kb) {
-                double acc = 0.0;
-                for (int l = 0; l <= j; ++l) acc += pr[r][l] * w[j][l];
-                P[(row0 + r) + (int64_t)j * lda] = acc;
-            }
+    for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) r[ii][jj] = q.S[(k0 + bi + ii) * np + k0 + bj + jj];
+    const int bad = gj_inverse64(r, rowb, colb);
+    if (I == K) {
+#pragma unroll
+        for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) q.D[(bi + ii) * CB + bj + jj] = r[ii][jj];
+        if (t == 0 && bad && *q.info == 0) *q.info = (int)k0 + bad;
+        return;
+    }
+#pragma unroll
+    for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            dl[bi + ii][bj + jj] = r[ii][jj];
+            sl[bi + ii][bj + jj] = q.S[(i0 + bi + ii) * np + k0 + bj + jj];
+            q.Rw[(int64_t)(bi + ii) * np + i0 + bj + jj] = q.S[(k0 + bi + ii) * np + i0 + bj + jj];
         }
-    }
-}
-
-// Trailing update C -= P P^T of the lower triangle (C m x m, P m x kb, both column-major with lda),
-// one 64 x 64 tile of C per workgroup (tiles on and below the diagonal, column by column), each lane
-// 4 x 4 entries; batched over blockIdx.y like k_chol_diag (problem y at P + y dA, C + y dA).
-__global__ __launch_bounds__(256) void k_chol_update(const double *__restrict__ P, double *__restrict__ C, int64_t lda,
-                                                     int m, int kb, int64_t dA) {
-    __shared__ double pi[CH_NB][CH_NB + 1];  // pi[r][k] = P(I0 + r, k)
-    __shared__ double pj[CH_NB][CH_NB + 1];  // pj[r][k] = P(J0 + r, k)
-    P += blockIdx.y * dA;
-    C += blockIdx.y * dA;
-    const int nT = (m + CH_NB - 1) / CH_NB;
-    int tile = blockIdx.x, J = 0;
-    while (tile >= nT - J) {
-        tile -= nT - J;
-        ++J;
-    }
-    const int I = J + tile;
-    const int I0 = I * CH_NB, J0 = J * CH_NB;
-    const int t = threadIdx.x;
-    for (int idx = t; idx < CH_NB * kb; idx += 256) {
-        const int r = idx % CH_NB, k = idx / CH_NB;
-        pi[r][k] = I0 + r < m ? P[(I0 + r) + (int64_t)k * lda] : 0.0;
-        pj[r][k] = J0 + r < m ? P[(J0 + r) + (int64_t)k * lda] : 0.0;
-    }
     __syncthreads();
-    const int ty = t % 16, tx = t / 16;  // rows I0 + 4 ty + i, columns J0 + 4 tx + j
     double acc[4][4] = {};
-    for (int k = 0; k < kb; ++k) {
+    for (int k = 0; k < CB; ++k) {
         double a[4], b[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) a[i] = pi[4 * ty + i][k];
+        for (int ii = 0; ii < 4; ++ii) a[ii] = sl[bi + ii][k];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) b[j] = pj[4 * tx + j][k];
+        for (int jj = 0; jj < 4; ++jj) b[jj] = dl[k][bj + jj];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int ii = 0; ii < 4; ++ii)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+            for (int jj = 0; jj < 4; ++jj) acc[ii][jj] += a[ii] * b[jj];
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int col = J0 + 4 * tx + j;
+    for (int ii = 0; ii < 4; ++ii)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int row = I0 + 4 * ty + i;
-            if (row < m && col < m && row >= col) C[row + (int64_t)col * lda] -= acc[i][j];
+        for (int jj = 0; jj < 4; ++jj) q.T[(i0 + bi + ii) * CB + bj + jj] = acc[ii][jj];
+}
+
+// Every 64 x 64 block (I, J) of the plane block from T, Rw and D.  The products run on the matrix cores
+// (MFMA f64 16x16x4: lane l holds A(l % 16, l / 16), B(l / 16, l % 16), result register r at
+// (4 r + l / 16, l % 16)); wave w computes the 32 x 32 quadrant (w / 2, w % 2) of the block.
+typedef double tg_cdbl4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_sweep_update(const SweepProb A, const SweepProb B, int K) {
+    __shared__ double al[CB][CB + 1];
+    __shared__ double bl[CB][CB + 1];
+    const SweepProb &q = blockIdx.y ? B : A;
+    if (K >= q.nb) return;
+    const int I = blockIdx.x / q.nb, J = blockIdx.x % q.nb;
+    if (I >= q.nb) return;
+    const int t = threadIdx.x;
+    const int64_t np = q.np, i0 = (int64_t)I * CB, j0 = (int64_t)J * CB;
+    if (J == K) {  // S_IK <- T_I, S_KK <- D
+        for (int idx = t; idx < CB * CB; idx += 256) {
+            const int ii = idx / CB, jj = idx % CB;
+            q.S[(i0 + ii) * np + j0 + jj] = I == K ? q.D[ii * CB + jj] : q.T[(i0 + ii) * CB + jj];
         }
+        return;
     }
+    // S_IJ -= T_I Rw_J  (I != K),  S_KJ = -D Rw_J
+    for (int idx = t; idx < CB * CB; idx += 256) {
+        const int ii = idx / CB, jj = idx % CB;
+        al[ii][jj] = I == K ? q.D[ii * CB + jj] : q.T[(i0 + ii) * CB + jj];
+        bl[ii][jj] = q.Rw[(int64_t)ii * np + j0 + jj];
+    }
+    __syncthreads();
+    const int w = t >> 6, lane = t & 63, li = lane & 15, lk = lane >> 4;
+    const int R0 = 32 * (w >> 1), C0 = 32 * (w & 1);
+    tg_cdbl4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = tg_cdbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int kk = 0; kk < CB / 4; ++kk) {
+        const double x0 = al[R0 + li][4 * kk + lk], x1 = al[R0 + 16 + li][4 * kk + lk];
+        const double y0 = bl[4 * kk + lk][C0 + li], y1 = bl[4 * kk + lk][C0 + 16 + li];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, y1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, y1, acc[1][1], 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                double *dst = q.S + (i0 + R0 + 16 * a + 4 * r + lk) * np + j0 + C0 + 16 * b + li;
+                *dst = I == K ? -acc[a][b][r] : *dst - acc[a][b][r];
+            }
 }
 
-__global__ void k_identity(double *__restrict__ X, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * n) return;
-    X[i] = (i % n == i / n) ? 1.0 : 0.0;
+// ---- solve operators:  Out(i, col0 + 3 jn + c) = sum_k S(i, C(3 jn, k)) V(3 jn + c, k) ----------------
+// (S^-1 times a coupling block whose column 3 jn + c holds the 27 couplings of the neighbouring plane's
+// DOF 3 jn + c; the three DOFs of one node share their column pattern.)
+struct DSProb {
+    const double *S;
+    int64_t np, nrows, ncn;  // S row stride, rows, column nodes
+    const double *V;
+    const int32_t *C;
+    double *Out;
+    int64_t ld, col0;
+};
+
+__global__ __launch_bounds__(256) void k_dense_sparse(const DSProb P0, const DSProb P1, const DSProb P2,
+                                                      const DSProb P3) {
+    const DSProb &q = blockIdx.y == 0 ? P0 : (blockIdx.y == 1 ? P1 : (blockIdx.y == 2 ? P2 : P3));
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= q.nrows * q.ncn) return;
+    const int64_t i = t / q.ncn, jn = t - i * q.ncn;
+    const double *srow = q.S + i * q.np;
+    const int32_t *C = q.C + 3 * jn * NCPL;
+    const double *V = q.V + 3 * jn * NCPL;
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+#pragma unroll 9
+    for (int k = 0; k < NCPL; ++k) {
+        const double sv = srow[C[k]];
+        o0 += sv * V[k];
+        o1 += sv * V[NCPL + k];
+        o2 += sv * V[2 * NCPL + k];
+    }
+    double *o = q.Out + i * q.ld + q.col0 + 3 * jn;
+    o[0] = o0;
+    o[1] = o1;
+    o[2] = o2;
 }
 
-// Row-major copy of a column-major n x n block: Tr[r * n + c] = A[r + n * c].
-__global__ void k_transpose_block(const double *__restrict__ A, double *__restrict__ Tr, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n * n) return;
-    const int64_t r = i / n, c = i - r * n;
-    Tr[i] = A[r + n * c];
-}
-
-// Batched triangular products over all planes, out_p = Tri_p * in_p with Tri_p row-major n_p x n_p,
-// lower (columns 0..r) or upper (columns r..n_p-1) triangular.  One wavefront per pair of rows
-// (r, n_p - 1 - r) of a plane (pair Q, plane pairPlane[Q], r = Q - pairOff[p]; the middle row of an
-// odd n_p alone): the pair holds n_p + 1 entries, so every wave streams the same amount.  The pair's
-// two row segments are walked as one run of n_p + 1 entries, 8 x 64 per round with every load issued
-// before the first use (clamped indices).
-__global__ __launch_bounds__(256) void k_tri_pairs(const double *__restrict__ Tri, const int64_t *__restrict__ adOff,
-                                                   const int64_t *__restrict__ off, const int32_t *__restrict__ pairPlane,
-                                                   const int64_t *__restrict__ pairOff, int64_t npairs, int upper,
-                                                   const double *__restrict__ in, double *__restrict__ out) {
+// c = blockdiag(S_p^-1) b over all planes: one wavefront per DOF row.
+__global__ __launch_bounds__(256) void k_plane_gemv(const double *__restrict__ S, const int64_t *__restrict__ sOff,
+                                                    const int64_t *__restrict__ npad, const int64_t *__restrict__ off,
+                                                    const int32_t *__restrict__ rowPlane, int64_t ntot,
+                                                    const double *__restrict__ b, double *__restrict__ c) {
     const int lane = threadIdx.x & 63;
-    const int64_t Q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (Q >= npairs) return;
-    const int p = pairPlane[Q];
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= ntot) return;
+    const int p = rowPlane[r];
     const int64_t o = off[p], n = off[p + 1] - o;
-    const int64_t r1 = Q - pairOff[p], r2 = n - 1 - r1;
-    const int64_t len1 = upper ? n - r1 : r1 + 1;
-    const int64_t len2 = r2 == r1 ? 0 : (upper ? n - r2 : r2 + 1);
-    const int64_t c1 = upper ? r1 : 0, c2 = upper ? r2 : 0;
-    const double *row1 = Tri + adOff[p] + r1 * n + c1, *x1 = in + o + c1;
-    const double *row2 = Tri + adOff[p] + r2 * n + c2 - len1, *x2 = in + o + c2 - len1;  // indexed by j >= len1
-    const int64_t tot = len1 + len2;
-    double s1[8] = {0, 0, 0, 0, 0, 0, 0, 0}, s2[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int64_t j0 = lane; j0 < tot; j0 += 8 * 64) {
-        double a[8], b[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int64_t j = min(j0 + 64 * k, tot - 1);
-            const bool first = j < len1;
-            a[k] = first ? row1[j] : row2[j];
-            b[k] = first ? x1[j] : x2[j];
-        }
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const int64_t j = j0 + 64 * k;
-            const double v = j < tot ? a[k] * b[k] : 0.0;
-            if (j < len1) s1[k] += v;
-            else s2[k] += v;
-        }
+    const double *row = S + sOff[p] + (r - o) * npad[p];
+    const double *x = b + o;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t j = lane;
+    for (; j + 192 < n; j += 256) {
+        const double a0 = row[j], a1 = row[j + 64], a2 = row[j + 128], a3 = row[j + 192];
+        const double x0 = x[j], x1 = x[j + 64], x2 = x[j + 128], x3 = x[j + 192];
+        s[0] += a0 * x0;
+        s[1] += a1 * x1;
+        s[2] += a2 * x2;
+        s[3] += a3 * x3;
     }
-    double t1 = ((s1[0] + s1[1]) + (s1[2] + s1[3])) + ((s1[4] + s1[5]) + (s1[6] + s1[7]));
-    double t2 = ((s2[0] + s2[1]) + (s2[2] + s2[3])) + ((s2[4] + s2[5]) + (s2[6] + s2[7]));
-    for (int o2 = 32; o2 > 0; o2 >>= 1) {
-        t1 += __shfl_down(t1, o2, 64);
-        t2 += __shfl_down(t2, o2, 64);
-    }
-    if (lane == 0) {
-        out[o + r1] = t1;
-        if (len2) out[o + r2] = t2;
-    }
+    for (; j < n; j += 64) s[0] += row[j] * x[j];
+    double tsum = (s[0] + s[1]) + (s[2] + s[3]);
+    for (int o2 = 32; o2 > 0; o2 >>= 1) tsum += __shfl_down(tsum, o2, 64);
+    if (lane == 0) c[r] = tsum;
 }
 
 // One step of a block recurrence, one wavefront per row:  y[r] = c[r] - sum_j M[r * ld + j] v(j), with
@@ -456,17 +465,10 @@ __global__ void k_unpack_planes(const double *__restrict__ b, T *__restrict__ f,
     f[3 * n + 2] = (T)b[base + 2];
 }
 
-void rb(rocblas_status s, const char *what) {
-    if (s != rocblas_status_success)
-        fail(TG_ERR_HIP, "tg_solver:rocblas", "%s failed with rocblas_status %d", what, (int)s);
-}
-
 }  // namespace
 
 CoarseSolver::~CoarseSolver() {
     if (exec) (void)hipGraphExecDestroy(exec);
-    if (h) rocblas_destroy_handle(h);
-
 }
 
 void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
@@ -490,23 +492,25 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
                 lid[n] = (int32_t)count[p]++;
             }
     nd.assign(P, 0);
+    npad.assign(P, 0);
     off.assign(P + 1, 0);
+    sOff.assign(P + 1, 0);
+    int64_t maxpad = CB;
     for (int p = 0; p < P; ++p) {
         nd[p] = 3 * count[p];
+        npad[p] = ((nd[p] + CB - 1) / CB) * CB;
+        maxpad = std::max(maxpad, npad[p]);
         off[p + 1] = off[p] + nd[p];
+        sOff[p + 1] = sOff[p] + npad[p] * npad[p];
     }
     ntot = off[P];
     m = P / 2;
-    adOff.assign(P + 1, 0);
-    asOff.assign(P + 1, 0);
     fOff.assign(P + 1, 0);
     bOff.assign(P + 1, 0);
     fLd.assign(P, 0);
     bLd.assign(P, 0);
     for (int p = 0; p < P; ++p) {
         const int64_t prev = p > 0 ? nd[p - 1] : 0, next = p + 1 < P ? nd[p + 1] : 0;
-        adOff[p + 1] = adOff[p] + nd[p] * nd[p];
-        asOff[p + 1] = asOff[p] + nd[p] * prev;
         // forward operator columns: the plane it is eliminated against (both neighbours for m)
         const int64_t fc = p < m ? prev : (p > m ? next : prev + next);
         // backward operator columns: the plane it is back-substituted from (none for m)
@@ -516,197 +520,180 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
         fOff[p + 1] = fOff[p] + nd[p] * fLd[p];
         bOff[p + 1] = bOff[p] + nd[p] * bLd[p];
     }
-    // row pairs (r, n_p - 1 - r) of the batched triangular products
-    std::vector<int64_t> pOff(P + 1, 0);
-    for (int p = 0; p < P; ++p) pOff[p + 1] = pOff[p] + (nd[p] + 1) / 2;
-    npairs = pOff[P];
-    std::vector<int32_t> pp(std::max<int64_t>(npairs, 1), 0);
+    std::vector<int32_t> rp(std::max<int64_t>(ntot, 1), 0);
     for (int p = 0; p < P; ++p)
-        for (int64_t Q = pOff[p]; Q < pOff[p + 1]; ++Q) pp[Q] = p;
+        for (int64_t r = off[p]; r < off[p + 1]; ++r) rp[r] = p;
     lidx.upload(lid, st);
     d_off.upload(off, st);
-    d_adOff.upload(adOff, st);
-    d_asOff.upload(asOff, st);
-    pairPlane.upload(pp, st);
-    d_pairOff.upload(pOff, st);
-    Ad.alloc(adOff[P]);
-    Linv.alloc(adOff[P]);
-    LT.alloc(adOff[P]);
-    As.alloc(std::max<int64_t>(asOff[P], 1));
+    d_sOff.upload(sOff, st);
+    d_npad.upload(npad, st);
+    d_nd.upload(nd, st);
+    rowPlane.upload(rp, st);
+    Sinv.alloc(std::max<int64_t>(sOff[P], 1));
+    dnV.alloc(std::max<int64_t>(ntot * NCPL, 1));
+    upV.alloc(std::max<int64_t>(ntot * NCPL, 1));
+    dnC.alloc(std::max<int64_t>(ntot * NCPL, 1));
+    upC.alloc(std::max<int64_t>(ntot * NCPL, 1));
     Fop.alloc(std::max<int64_t>(fOff[P], 1));
     Bop.alloc(std::max<int64_t>(bOff[P], 1));
     Fop.zero(st);  // row padding stays zero
     Bop.zero(st);
-    bb.alloc(ntot);
-    yy.alloc(ntot);
-    xx.alloc(ntot);
+    T.alloc(2 * maxpad * CB);
+    Rw.alloc(2 * maxpad * CB);
+    Dk.alloc(2 * CB * CB);
+    bb.alloc(std::max<int64_t>(ntot, 1));
+    yy.alloc(std::max<int64_t>(ntot, 1));
     info.alloc(P);
-    chW.alloc(2 * CH_NB * CH_NB);
-    if (!h) rb(rocblas_create_handle(&h), "rocblas_create_handle");
     TG_HIP(hipStreamSynchronize(st), "tg_solver:coarse");
 }
 
 void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask, const uint8_t *nodeMask,
                           hipStream_t st) {
     const char *id = "tg_solver:coarse";
-    rb(rocblas_set_stream(h, st), "rocblas_set_stream");
-    Ad.zero(st);
-    if (asOff[P]) As.zero(st);
+    factored = false;
+    int64_t maxpad = CB, maxn = 1;
+    for (int p = 0; p < P; ++p) {
+        maxpad = std::max(maxpad, npad[p]);
+        maxn = std::max(maxn, nd[p]);
+    }
+    k_plane_init<<<dim3(grid_for(maxpad * maxpad, 256), P), 256, 0, st>>>(Sinv, d_sOff, d_npad, d_nd);
     info.zero(st);
-    const int64_t nth = g.nodes() * 243;
-    k_coarse_assemble<<<grid_for(nth, 256), 256, 0, st>>>(Ks, ldk, eleMask, nodeMask, g, axis, lidx, d_off, d_adOff,
-                                                          d_asOff, Ad, As);
+    k_coarse_assemble<<<grid_for(g.nodes() * 243, 256), 256, 0, st>>>(Ks, ldk, eleMask, nodeMask, g, axis, lidx, d_off,
+                                                                      d_sOff, d_npad, Sinv, dnV, dnC, upV, upC);
     TG_LAUNCH_CHECK(id);
-    const double one = 1.0, mone = -1.0, zero = 0.0;
-    // As_p (n_p x n_{p-1}, column-major) holds A_{p,p-1}.  Chain A turns As_{p+1} into
-    // L_{p+1,p} = A_{p+1,p} L_pp^-T (right trsm); chain B turns As_p into X_p = L_pp^-1 A_{p,p-1}
-    // = L_{p-1,p}^T (left trsm).
-    // Plane factor (blocked Cholesky, kernels above) and its inverse inv(L) = L \ I by one triangular solve.  Every
-    // call runs on the solver stream: with the two chains on two streams (two rocBLAS handles) the
-    // factors came out different from run to run on ROCm 7.2 (wrong in the low to leading digits on
-    // about one assembly in eight), however the streams were ordered; on one stream they are
-    // bitwise reproducible.
-    // Factor plane p, or planes p and q (same size) together: the LDS kernels take both in one launch
-    // (blockIdx.y); the rocBLAS calls stay per plane.
-    auto factor_planes = [&](int p, int q) {
-        const rocblas_int np = (rocblas_int)nd[p];
-        const int nb = q >= 0 ? 2 : 1;
-        const int64_t dA = q >= 0 ? adOff[q] - adOff[p] : 0;
-        const int dinfo = q >= 0 ? q - p : 0;
-        for (int k0 = 0; k0 < np; k0 += CH_NB) {
-            const int kb = std::min<int>(CH_NB, np - k0), mr = np - k0 - kb;
-            double *D = Ad.p + adOff[p] + k0 + (int64_t)k0 * np;
-            k_chol_diag<<<dim3(1, nb), 256, 0, st>>>(D, np, kb, chW.p, info.p + p, k0, dA, dinfo);
-            if (mr > 0)
-                k_chol_panel<<<dim3((mr + CH_PR - 1) / CH_PR, nb), 256, 0, st>>>(D + kb, np, mr, kb, chW.p, dA);
-            TG_LAUNCH_CHECK(id);
-            if (mr > 0) {
-                const int nT = (mr + CH_NB - 1) / CH_NB;
-                k_chol_update<<<dim3(nT * (nT + 1) / 2, nb), 256, 0, st>>>(D + kb, D + kb + (int64_t)kb * np, np, mr,
-                                                                        kb, dA);
-                TG_LAUNCH_CHECK(id);
+    auto S_of = [&](int p) { return Sinv.p + sOff[p]; };
+    auto has = [&](int p) { return p >= 0 && p < P && nd[p] > 0; };
+    const SchurProb noneS{};
+    // S_p -= A_{p,p-1} B_{p-1} (chain A, dn rows of p) and / or A_{p,p+1} B_{p+1} (chain B, up rows of p)
+    auto schur = [&](int p, bool withPrev, bool withNext) {
+        SchurProb q{};
+        q.S = S_of(p);
+        q.np = npad[p];
+        q.n = nd[p];
+        if (withPrev && has(p - 1))
+            q.t[q.nt++] = SchurTerm{dnV.p + off[p] * NCPL, dnC.p + off[p] * NCPL, Bop.p + bOff[p - 1], bLd[p - 1]};
+        if (withNext && has(p + 1))
+            q.t[q.nt++] = SchurTerm{upV.p + off[p] * NCPL, upC.p + off[p] * NCPL, Bop.p + bOff[p + 1], bLd[p + 1]};
+        return q;
+    };
+    auto launch_schur = [&](const SchurProb &A, const SchurProb &B) {
+        if (!A.nt && !B.nt) return;
+        const int64_t n = std::max(A.nt ? A.n : 0, B.nt ? B.n : 0);
+        k_schur<<<dim3(grid_for(n, 256), (unsigned)n, 2), 256, 0, st>>>(A.nt ? A : noneS, B.nt ? B : noneS);
+        TG_LAUNCH_CHECK(id);
+    };
+    // S_p <- S_p^-1 for up to two planes together
+    auto sweep = [&](int p, int q) {
+        SweepProb pr[2] = {};
+        int nbmax = 0;
+        const int pl[2] = {p, q};
+        for (int y = 0; y < 2; ++y) {
+            if (!has(pl[y])) continue;
+            pr[y] = SweepProb{S_of(pl[y]), npad[pl[y]], (int)(npad[pl[y]] / CB), T.p + y * maxpad * CB,
+                              Rw.p + y * maxpad * CB, Dk.p + y * CB * CB, info.p + pl[y]};
+            nbmax = std::max(nbmax, pr[y].nb);
+        }
+        for (int K = 0; K < nbmax; ++K) {
+            k_sweep_panel<<<dim3(nbmax, 2), 256, 0, st>>>(pr[0], pr[1], K);
+            k_sweep_update<<<dim3(nbmax * nbmax, 2), 256, 0, st>>>(pr[0], pr[1], K);
+        }
+        TG_LAUNCH_CHECK(id);
+    };
+    // solve operators: Out = S_p^-1 A_{p,r} for neighbour plane r, from r's coupling rows towards p
+    std::vector<DSProb> ops;
+    auto op = [&](int p, int r, double *out, int64_t ld, int64_t col0) {
+        if (!has(p) || !has(r)) return;
+        const bool rIsNext = r == p + 1;  // A_{p,p+1}(., j) = A_{p+1,p}(j, .) = dn rows of p+1; else up rows of p-1
+        DSProb d;
+        d.S = S_of(p);
+        d.np = npad[p];
+        d.nrows = nd[p];
+        d.ncn = nd[r] / 3;
+        d.V = (rIsNext ? dnV.p : upV.p) + off[r] * NCPL;
+        d.C = (rIsNext ? dnC.p : upC.p) + off[r] * NCPL;
+        d.Out = out;
+        d.ld = ld;
+        d.col0 = col0;
+        ops.push_back(d);
+    };
+    auto flush_ops = [&]() {
+        for (size_t i0 = 0; i0 < ops.size(); i0 += 4) {
+            DSProb q[4] = {};
+            int64_t work = 0;
+            const int cnt = (int)std::min<size_t>(4, ops.size() - i0);
+            for (int y = 0; y < cnt; ++y) {
+                q[y] = ops[i0 + y];
+                work = std::max(work, q[y].nrows * q[y].ncn);
             }
+            if (work) k_dense_sparse<<<dim3(grid_for(work, 256), cnt), 256, 0, st>>>(q[0], q[1], q[2], q[3]);
         }
-        for (int y = 0; y < nb; ++y) {
-            const int pp = y ? q : p;
-            k_identity<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Linv.p + adOff[pp], np);
-            TG_LAUNCH_CHECK(id);
-            rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                             rocblas_diagonal_non_unit, np, np, &one, Ad.p + adOff[pp], np, Linv.p + adOff[pp], np),
-               "rocblas_dtrsm inverse");
-        }
+        TG_LAUNCH_CHECK(id);
+        ops.clear();
     };
     // chains A (p = 0 .. m-1) and B (q = P-1 .. m+1) advance together, one plane each per step
     const int lenA = m, lenB = P - 1 - m;
     for (int k = 0; k < std::max(lenA, lenB); ++k) {
-        const int p = k < lenA && nd[k] > 0 ? k : -1;
-        const int q = k < lenB && nd[P - 1 - k] > 0 ? P - 1 - k : -1;
-        if (p > 0 && nd[p - 1] > 0)  // S_p -= L_{p,p-1} L_{p,p-1}^T
-            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, (rocblas_int)nd[p], (rocblas_int)nd[p - 1],
-                             &mone, As.p + asOff[p], (rocblas_int)nd[p], &one, Ad.p + adOff[p], (rocblas_int)nd[p]),
-               "rocblas_dsyrk");
-        if (q >= 0 && q + 1 < P && nd[q + 1] > 0)  // S_q -= X_{q+1}^T X_{q+1}
-            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, (rocblas_int)nd[q],
-                             (rocblas_int)nd[q + 1], &mone, As.p + asOff[q + 1], (rocblas_int)nd[q + 1], &one,
-                             Ad.p + adOff[q], (rocblas_int)nd[q]),
-               "rocblas_dsyrk");
-        if (p >= 0 && q >= 0 && nd[p] == nd[q]) {
-            factor_planes(p, q);
-        } else {
-            if (p >= 0) factor_planes(p, -1);
-            if (q >= 0) factor_planes(q, -1);
+        const int p = k < lenA && has(k) ? k : -1;
+        const int q = k < lenB && has(P - 1 - k) ? P - 1 - k : -1;
+        launch_schur(p > 0 ? schur(p, true, false) : noneS, q >= 0 && q < P - 1 ? schur(q, false, true) : noneS);
+        sweep(p, q);
+        if (p >= 0) {
+            op(p, p + 1, Bop.p + bOff[p], bLd[p], 0);  // B_p = S_p^-1 A_{p,p+1}
+            op(p, p - 1, Fop.p + fOff[p], fLd[p], 0);  // F_p = S_p^-1 A_{p,p-1}
         }
-        if (p >= 0 && nd[p + 1] > 0)  // L_{p+1,p} = A_{p+1,p} L_pp^-T
-            rb(rocblas_dtrsm(h, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
-                             rocblas_diagonal_non_unit, (rocblas_int)nd[p + 1], (rocblas_int)nd[p], &one,
-                             Ad.p + adOff[p], (rocblas_int)nd[p], As.p + asOff[p + 1], (rocblas_int)nd[p + 1]),
-               "rocblas_dtrsm A");
-        if (q >= 0 && nd[q - 1] > 0)  // X_q = L_qq^-1 A_{q,q-1}
-            rb(rocblas_dtrsm(h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                             rocblas_diagonal_non_unit, (rocblas_int)nd[q], (rocblas_int)nd[q - 1], &one,
-                             Ad.p + adOff[q], (rocblas_int)nd[q], As.p + asOff[q], (rocblas_int)nd[q]),
-               "rocblas_dtrsm B");
+        if (q >= 0) {
+            op(q, q - 1, Bop.p + bOff[q], bLd[q], 0);  // B_q = S_q^-1 A_{q,q-1}
+            op(q, q + 1, Fop.p + fOff[q], fLd[q], 0);  // F_q = S_q^-1 A_{q,q+1}
+        }
+        flush_ops();
     }
-    if (nd[m] > 0) {  // separator
-        const rocblas_int nm = (rocblas_int)nd[m];
-        if (m > 0 && nd[m - 1] > 0)
-            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_none, nm, (rocblas_int)nd[m - 1], &mone,
-                             As.p + asOff[m], nm, &one, Ad.p + adOff[m], nm), "rocblas_dsyrk");
-        if (m + 1 < P && nd[m + 1] > 0)
-            rb(rocblas_dsyrk(h, rocblas_fill_lower, rocblas_operation_transpose, nm, (rocblas_int)nd[m + 1], &mone,
-                             As.p + asOff[m + 1], (rocblas_int)nd[m + 1], &one, Ad.p + adOff[m], nm), "rocblas_dsyrk");
-        factor_planes(m, -1);
+    if (has(m)) {  // separator
+        launch_schur(schur(m, true, true), noneS);
+        sweep(m, -1);
+        const int64_t na = has(m - 1) ? nd[m - 1] : 0;
+        op(m, m - 1, Fop.p + fOff[m], fLd[m], 0);
+        op(m, m + 1, Fop.p + fOff[m], fLd[m], na);
+        flush_ops();
     }
-    // Solve operators, row-major with zero-padded rows (column-major results of transposed products):
-    //   LT_p = inv(L_pp) (row-major copy); inv(L_pp)^T is the column-major Linv itself.
-    //   forward  A: inv(L_pp) L_{p,p-1}       = (As_p^T Linv_p^T)^T      (n_p x n_{p-1})
-    //            B: inv(L_pp) L_{p,p+1}       = (X_{p+1} Linv_p^T)^T     (n_p x n_{p+1})
-    //            m: [inv(L_mm) L_{m,m-1} | inv(L_mm) L_{m,m+1}]
-    //   backward A: inv(L_pp)^T L_{p+1,p}^T   = (As_{p+1} Linv_p)^T      (n_p x n_{p+1})
-    //            B: inv(L_pp)^T L_{p-1,p}^T   = (X_p^T Linv_p)^T         (n_p x n_{p-1})
-    for (int p = 0; p < P; ++p) {
-        const rocblas_int np = (rocblas_int)nd[p];
-        if (np == 0) continue;
-        const int64_t prev = p > 0 ? nd[p - 1] : 0, next = p + 1 < P ? nd[p + 1] : 0;
-        const double *Lp = Linv.p + adOff[p];
-        k_transpose_block<<<grid_for((int64_t)np * np, 256), 256, 0, st>>>(Lp, LT.p + adOff[p], np);
-        TG_LAUNCH_CHECK(id);
-        double *F = Fop.p + fOff[p], *B = Bop.p + bOff[p];
-        const rocblas_int fl = (rocblas_int)fLd[p], bl = (rocblas_int)bLd[p];
-        if (p <= m && prev > 0)  // A-side coupling (chain A, or the separator's first block)
-            rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_transpose, (rocblas_int)prev, np, np,
-                             &one, As.p + asOff[p], np, Lp, np, &zero, F, fl), "rocblas_dgemm F A");
-        if (p >= m && next > 0)  // B-side coupling (chain B, or the separator's second block)
-            rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, (rocblas_int)next, np, np,
-                             &one, As.p + asOff[p + 1], (rocblas_int)next, Lp, np, &zero, F + (p == m ? prev : 0),
-                             fl), "rocblas_dgemm F B");
-        if (p < m && next > 0)
-            rb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)next, np, np, &one,
-                             As.p + asOff[p + 1], (rocblas_int)next, Lp, np, &zero, B, bl), "rocblas_dgemm B A");
-        if (p > m && prev > 0)
-            rb(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)prev, np, np, &one,
-                             As.p + asOff[p], np, Lp, np, &zero, B, bl), "rocblas_dgemm B B");
-    }
-    std::vector<rocblas_int> hinfo(P);
-    TG_HIP(hipMemcpyAsync(hinfo.data(), info.p, sizeof(rocblas_int) * P, hipMemcpyDeviceToHost, st), id);
+    std::vector<int> hinfo(P);
+    TG_HIP(hipMemcpyAsync(hinfo.data(), info.p, sizeof(int) * P, hipMemcpyDeviceToHost, st), id);
     TG_HIP(hipStreamSynchronize(st), id);
     for (int p = 0; p < P; ++p)
         if (hinfo[p] != 0)
             fail(TG_ERR_NUMERIC, "tg_solver:coarse",
-                 "coarsest-level factorisation failed on plane block %d (info %d): matrix not positive definite",
-                 p, (int)hinfo[p]);
+                 "coarsest-level factorisation failed on plane block %d (pivot %d): matrix not positive definite",
+                 p, hinfo[p]);
     factored = true;
 }
 
-// The solve's fixed sequence on the internal plane-ordered vectors (bb -> xx):
-//   yy = blockdiag(LT) bb; forward chains A (p = 1 .. m-1) and B (p = P-2 .. m+1) step by step together;
-//   the separator; xx = blockdiag(Linv^T) yy; backward chains A (p = m-1 .. 0) and B (p = m+1 .. P-1).
+// The solve's fixed sequence on the internal plane-ordered vectors (bb -> yy, in place on yy):
+//   yy = blockdiag(S_p^-1) bb; forward chains A (p = 1 .. m-1) and B (p = P-2 .. m+1) step by step together;
+//   the separator; backward chains A (p = m-1 .. 0) and B (p = m+1 .. P-1).
 void CoarseSolver::enqueue_core(hipStream_t s) const {
     const char *id = "tg_solver:coarse";
-    const unsigned gr = grid_for(npairs, 4);
     const ChainProb none = chain_prob(nullptr, 0, 0, 0, 0, nullptr, nullptr, nullptr, nullptr);
-    auto fwd = [&](int p, int q) -> ChainProb {  // y_p -= Fop_p y_q
-        if (p < 0 || p >= P || nd[p] == 0 || q < 0 || q >= P || nd[q] == 0) return none;
+    auto ok = [&](int p) { return p >= 0 && p < P && nd[p] > 0; };
+    auto fwd = [&](int p, int q) -> ChainProb {  // z_p = c_p - F_p z_q
+        if (!ok(p) || !ok(q)) return none;
         return chain_prob(Fop.p + fOff[p], nd[p], nd[q], nd[q], fLd[p], yy.p + off[p], yy.p + off[q], nullptr,
                           yy.p + off[p]);
     };
-    auto bwd = [&](int p, int q) -> ChainProb {  // x_p -= Bop_p x_q
-        if (p < 0 || p >= P || nd[p] == 0 || q < 0 || q >= P || nd[q] == 0) return none;
-        return chain_prob(Bop.p + bOff[p], nd[p], nd[q], nd[q], bLd[p], xx.p + off[p], xx.p + off[q], nullptr,
-                          xx.p + off[p]);
+    auto bwd = [&](int p, int q) -> ChainProb {  // x_p = z_p - B_p x_q
+        if (!ok(p) || !ok(q)) return none;
+        return chain_prob(Bop.p + bOff[p], nd[p], nd[q], nd[q], bLd[p], yy.p + off[p], yy.p + off[q], nullptr,
+                          yy.p + off[p]);
     };
-    k_tri_pairs<<<gr, 256, 0, s>>>(LT, d_adOff, d_off, pairPlane, d_pairOff, npairs, 0, bb, yy);
+    if (ntot) k_plane_gemv<<<grid_for(ntot, 4), 256, 0, s>>>(Sinv, d_sOff, d_npad, d_off, rowPlane, ntot, bb, yy);
     const int lenA = m, lenB = P - 1 - m;  // planes per chain
     for (int k = 1; k < std::max(lenA, lenB); ++k)
         launch_chain(k < lenA ? fwd(k, k - 1) : none, k < lenB ? fwd(P - 1 - k, P - k) : none, s);
-    if (nd[m] > 0) {  // separator: y_m -= Fop_m [y_{m-1}; y_{m+1}]
-        const int64_t na = (m > 0) ? nd[m - 1] : 0, nb = (m + 1 < P) ? nd[m + 1] : 0;
+    if (ok(m)) {  // separator: x_m = c_m - F_m [z_{m-1}; z_{m+1}]
+        const int64_t na = ok(m - 1) ? nd[m - 1] : 0, nb = ok(m + 1) ? nd[m + 1] : 0;
         if (na + nb > 0)
             launch_chain(chain_prob(Fop.p + fOff[m], nd[m], na + nb, na, fLd[m], yy.p + off[m],
                                     na ? yy.p + off[m - 1] : yy.p, nb ? yy.p + off[m + 1] : yy.p, yy.p + off[m]),
                          none, s);
     }
-    k_tri_pairs<<<gr, 256, 0, s>>>(Linv, d_adOff, d_off, pairPlane, d_pairOff, npairs, 1, yy, xx);
     for (int k = 1; k <= std::max(lenA, lenB); ++k)
         launch_chain(k <= lenA ? bwd(m - k, m - k + 1) : none, k <= lenB ? bwd(m + k, m + k - 1) : none, s);
     TG_LAUNCH_CHECK(id);
@@ -722,20 +709,20 @@ void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t 
         enqueue_core(st);
     } else {
         if (!exec) {
-        // the sequence reads only buffers owned here, so one capture serves every later solve
-        hipStream_t cap;
-        TG_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), id);
-        TG_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed), id);
-        enqueue_core(cap);
-        hipGraph_t graph;
-        TG_HIP(hipStreamEndCapture(cap, &graph), id);
-        TG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), id);
-        TG_HIP(hipGraphDestroy(graph), id);
-        TG_HIP(hipStreamDestroy(cap), id);
+            // the sequence reads only buffers owned here, so one capture serves every later solve
+            hipStream_t cap;
+            TG_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), id);
+            TG_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed), id);
+            enqueue_core(cap);
+            hipGraph_t graph;
+            TG_HIP(hipStreamEndCapture(cap, &graph), id);
+            TG_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), id);
+            TG_HIP(hipGraphDestroy(graph), id);
+            TG_HIP(hipStreamDestroy(cap), id);
+        }
+        TG_HIP(hipGraphLaunch(exec, st), id);
     }
-    TG_HIP(hipGraphLaunch(exec, st), id);
-    }
-    k_unpack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(xx, x, nodeMask, lidx, d_off, g, axis);
+    k_unpack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(yy, x, nodeMask, lidx, d_off, g, axis);
     TG_LAUNCH_CHECK(id);
 }
 

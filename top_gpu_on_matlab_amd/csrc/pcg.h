@@ -12,21 +12,21 @@ namespace tg {
 
 enum { S_PV = 0, S_RR = 1, S_BB = 2, S_XA = 3, S_XB = 4 };
 
-// y += lambda p; r -= lambda v; partial sums of r.r  (TOP_GPU.m:630-632, 637).  R32: also store r
-// rounded to FP32 (the mixed V-cycle's input, TOP_GPU.m:775-777), saving a conversion pass.
+// r -= lambda v; partial sums of r.r  (TOP_GPU.m:631-632, 637).  The iterate's update y += lambda p
+// (:630) is applied later, by the next direction update, which reads p anyway (or by k_pcg_axpy when
+// the loop stops): the same operation on the same values, one pass over p and y fewer per iteration.
+// R32: also store r rounded to FP32 (the mixed V-cycle's input, TOP_GPU.m:775-777), saving a
+// conversion pass.
 template <bool R32>
-__global__ __launch_bounds__(RED_THREADS) static void k_pcg_update(double *__restrict__ y, double *__restrict__ r,
-                                                                   const double *__restrict__ p,
-                                                                   const double *__restrict__ v, int64_t n,
-                                                                   const double *__restrict__ scal, int ix1, int ipv,
-                                                                   double *__restrict__ partial,
+__global__ __launch_bounds__(RED_THREADS) static void k_pcg_update(double *__restrict__ r, const double *__restrict__ v,
+                                                                   int64_t n, const double *__restrict__ scal, int ix1,
+                                                                   int ipv, double *__restrict__ partial,
                                                                    float *__restrict__ r32) {
     const double lam = scal[ix1] / scal[ipv];
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
     double s = 0;
     for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        y[i] = y[i] + lam * p[i];
         const double rr = r[i] - lam * v[i];
         r[i] = rr;
         if (R32) r32[i] = (float)rr;
@@ -36,12 +36,24 @@ __global__ __launch_bounds__(RED_THREADS) static void k_pcg_update(double *__res
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// p = z + (x2 / x1) p  (TOP_GPU.m:652); Z = float: z as the mixed V-cycle left it (widening exact)
+// y += (x1 / pv) p, then p = z + (x2 / x1) p  (TOP_GPU.m:630, 652): the previous iteration's update of
+// the iterate and the new direction; Z = float: z as the mixed V-cycle left it (widening exact)
 template <typename Z>
-__global__ static void k_pcg_direction(double *__restrict__ p, const Z *__restrict__ z, int64_t n,
-                                       const double *__restrict__ scal, int ix2, int ix1) {
+__global__ static void k_pcg_direction(double *__restrict__ p, double *__restrict__ y, const Z *__restrict__ z,
+                                       int64_t n, const double *__restrict__ scal, int ix2, int ix1, int ipv) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = (double)z[i] + scal[ix2] / scal[ix1] * p[i];
+    if (i < n) {
+        const double pi = p[i];
+        y[i] = y[i] + scal[ix1] / scal[ipv] * pi;
+        p[i] = (double)z[i] + scal[ix2] / scal[ix1] * pi;
+    }
+}
+
+// y += (x1 / pv) p: the last iteration's update of the iterate, when the loop stops on the residual
+__global__ static void k_pcg_axpy(double *__restrict__ y, const double *__restrict__ p, int64_t n,
+                                  const double *__restrict__ scal, int ix1, int ipv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = y[i] + scal[ix1] / scal[ipv] * p[i];
 }
 
 // p = (double) z32 (the first search direction, from an FP32 preconditioner output)
@@ -143,19 +155,23 @@ struct PcgWork {
         for (its = 1; its <= maxIT; ++its) {
             if (!atx(p.p, v.p, S_PV)) dot(p, v, S_PV);
             if (r32) {
-                k_pcg_update<true><<<RED_BLOCKS, RED_THREADS, 0, st>>>(y, r, p, v, n, scal, sx1, S_PV, partial, r32);
+                k_pcg_update<true><<<RED_BLOCKS, RED_THREADS, 0, st>>>(r, v, n, scal, sx1, S_PV, partial, r32);
                 r32_fresh = true;
             } else {
-                k_pcg_update<false><<<RED_BLOCKS, RED_THREADS, 0, st>>>(y, r, p, v, n, scal, sx1, S_PV, partial, nullptr);
+                k_pcg_update<false><<<RED_BLOCKS, RED_THREADS, 0, st>>>(r, v, n, scal, sx1, S_PV, partial, nullptr);
             }
             k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR);
             TG_LAUNCH_CHECK(id);
             res = std::sqrt(read(S_RR)) / normB;
             if (!std::isfinite(res)) fail(TG_ERR_NUMERIC, id, "non-finite residual at iteration %d", its);
-            if (res < tol) break;
+            if (res < tol) {
+                k_pcg_axpy<<<grid_for(n, 256), 256, 0, st>>>(y, p, n, scal, sx1, S_PV);
+                TG_LAUNCH_CHECK(id);
+                break;
+            }
             precond(sx2);
-            if (z32_fresh) k_pcg_direction<float><<<grid_for(n, 256), 256, 0, st>>>(p, z32, n, scal, sx2, sx1);
-            else k_pcg_direction<double><<<grid_for(n, 256), 256, 0, st>>>(p, z, n, scal, sx2, sx1);
+            if (z32_fresh) k_pcg_direction<float><<<grid_for(n, 256), 256, 0, st>>>(p, y, z32, n, scal, sx2, sx1, S_PV);
+            else k_pcg_direction<double><<<grid_for(n, 256), 256, 0, st>>>(p, y, z, n, scal, sx2, sx1, S_PV);
             TG_LAUNCH_CHECK(id);
             std::swap(sx1, sx2);
         }

@@ -49,6 +49,7 @@ struct Solver {
     size_t tused = 0;
     hipEvent_t *timing_slot(int key);
     int kfine_zchunk = 12;  // node planes per workgroup of the tiled kernel
+    int kfine_mfma = 1;     // FP32 tiled finest K*u: element products on the matrix cores (0: Walsh form)
     int kelem_parts = 24;   // threads per element in the per-element-matrix K*u
     bool has_modulus = false, assembled = false;
     std::vector<DevLevel> lv;
@@ -56,7 +57,7 @@ struct Solver {
     DevBuf<double> K0_64, E64, Esub, Imat, uFree, uFixed;
     DevBuf<float> K0_32, E32;
     DevBuf<int32_t> fullUid;
-    DevBuf<double> Gt, Ech;                // level-2 Galerkin GEMM: G_s' (nCh x 576), children moduli
+    DevBuf<double> Gl2, Ech;               // level-2 Galerkin GEMM: G_s (nCh x 576, entry fastest), children moduli
     DevBuf<int64_t> bndList;               // coarse elements with a boundary (fixed-DOF) child
     int64_t nBnd = 0;
     DevBuf<double> b, y;        // PCG right-hand side and iterate (full fine grid)

@@ -104,6 +104,52 @@ __global__ void k_gather_children(const double *__restrict__ E, const uint8_t *_
     Ech[ce + ldk * s] = fineSolid[f] ? E[f] : 0.0;
 }
 
+// Level-2 Galerkin product on the matrix cores: Ks[j * ldk + e] = sum_k Ech[e + ldk * k] G[k * 576 + j]
+// (k < 4 KS children, j < 576 entries), i.e. Ks_e = sum_s E_s (P_s' K0 P_s) for every coarse element.
+// Each MFMA f64 16x16x4 takes A = a 16-entry x 4-child block of G' and B = a 4-child x 16-element block of
+// Ech', so its 16 x 16 result is entries x elements and every store row is 16 consecutive elements.
+// Lane layouts of that instruction: lane l holds A(l % 16, l / 16), B(l / 16, l % 16) and result registers
+// r = 0..3 at D(4 r + l / 16, l % 16).  A
+// wavefront owns 16 NG consecutive elements, keeps their children moduli in registers and walks the 36
+// entry tiles; each G value loaded feeds NG MFMAs.  e ranges over the padded count ldk (a multiple of
+// 64; padded elements have zero moduli).
+typedef double tg_dbl4 __attribute__((ext_vector_type(4)));
+template <int KS, int NG>
+__global__ __launch_bounds__(256) void k_galerkin_l2_mfma(const double *__restrict__ Ech, const double *__restrict__ G,
+                                                          double *__restrict__ Ks, int64_t ldk) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int64_t e0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * NG);
+    if (e0 >= ldk) return;
+    double b[NG][KS];
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) b[g][kk] = Ech[(e0 + 16 * g + li) + ldk * (lk + 4 * kk)];
+    const double *gp = G + (int64_t)lk * 576 + li;
+    double a[KS];
+#pragma unroll
+    for (int kk = 0; kk < KS; ++kk) a[kk] = gp[(int64_t)4 * kk * 576];
+    for (int t = 0; t < 36; ++t) {
+        tg_dbl4 acc[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) acc[g] = tg_dbl4{0.0, 0.0, 0.0, 0.0};
+        double an[KS];  // next tile's G values, loaded while this tile's MFMAs run
+        const int tn = t + 1 < 36 ? t + 1 : t;
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) an[kk] = gp[(int64_t)4 * kk * 576 + 16 * tn];
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk)
+#pragma unroll
+            for (int g = 0; g < NG; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], b[g][kk], acc[g], 0, 0, 0);
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Ks[(int64_t)(16 * t + 4 * r + lk) * ldk + e0 + 16 * g + li] = acc[g][r];
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) a[kk] = an[kk];
+    }
+}
+
 // K_mat = I_mat * E_sub per element (TOP_GPU.m:1101-1103), entry-major output.
 __global__ void k_kmat(const double *__restrict__ Imat, const double *__restrict__ Esub, double *__restrict__ Km,
                        int64_t ldk, int64_t nE) {
@@ -297,8 +343,9 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
     for (int64_t k = 0; k < F.Ne; ++k) uid[m.lv[0].eleMapBack[k] - 1] = m.mapUniqueKes[k];
     fullUid.upload(uid, st);
     if (!super && L >= 2) {
-        // Level-2 Galerkin as a GEMM: Ks_e = sum_s E_s G_s with G_s = Pch_s' K0 Pch_s for children
-        // without fixed DOFs; coarse elements with a boundary child are recomputed child by child.
+        // Level-2 Galerkin as a GEMM on the matrix cores: Ks_e = sum_s E_s G_s with G_s = Pch_s' K0 Pch_s
+        // (linear in the children moduli) for children without fixed DOFs; coarse elements with a
+        // boundary child are recomputed child by child.
         DevLevel &C = lv[1];
         const int span = C.span, nCh = span * span * span;
         std::vector<double> Ph;
@@ -316,10 +363,10 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
                 for (int r = 0; r < 24; ++r) {
                     double v = 0;
                     for (int a = 0; a < 24; ++a) v += Pc[a + 24 * r] * tmp[a + 24 * c];
-                    gt[sc + (size_t)nCh * (r + 24 * c)] = v;
+                    gt[(size_t)576 * sc + (r + 24 * c)] = v;
                 }
         }
-        Gt.upload(gt, st);
+        Gl2.upload(gt, st);
         std::vector<int64_t> bnd;
         const Grid &cgd = C.g, &fgd = F.g;
         for (int64_t ce = 0; ce < cgd.elems(); ++ce) {
@@ -428,11 +475,14 @@ void Solver::assemble() {
             k_gather_children<<<grid_for(nC * nCh, 256), 256, 0, st>>>(E64, Fi.eleMask, C.span, C.g.EY, C.g.EX, nC,
                                                                       Fi.g.EY, Fi.g.EX, nCh, Ech, C.ldk);
             TG_LAUNCH_CHECK(id);
-            const double one = 1.0, zero = 0.0;
-            if (rocblas_set_stream(coarse.h, st) != rocblas_status_success ||
-                rocblas_dgemm(coarse.h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)nC, 576, nCh, &one,
-                              Ech, (rocblas_int)C.ldk, Gt, nCh, &zero, C.Ks64, (rocblas_int)C.ldk) != rocblas_status_success)
-                fail(TG_ERR_HIP, "tg_solver:rocblas", "level-2 Galerkin GEMM failed");
+            // ldk is a multiple of 64: 64 elements per wavefront, 256 per workgroup
+            if (nCh == 64)
+                k_galerkin_l2_mfma<16, 4><<<grid_for(C.ldk, 256), 256, 0, st>>>(Ech, Gl2, C.Ks64, C.ldk);
+            else if (nCh == 8)
+                k_galerkin_l2_mfma<2, 4><<<grid_for(C.ldk, 256), 256, 0, st>>>(Ech, Gl2, C.Ks64, C.ldk);
+            else
+                fail(TG_ERR_ARGS, id, "level-2 Galerkin: %d children per coarse element", nCh);
+            TG_LAUNCH_CHECK(id);
             if (nBnd) k_galerkin_list<<<(unsigned)nBnd, GK_THREADS, 0, st>>>(src, bndList, nCh, C.P, C.Ks64, C.ldk);
         } else if (i == 1) {
             SrcDevSuper src;
@@ -488,7 +538,20 @@ static void launch_kelem_vec(int parts, hipStream_t st, const T *u, T *fv, const
 template <typename T, bool CK, bool DOT>
 static void launch_walsh_k(int epi, unsigned gt, hipStream_t st, const T *u, T *out, const T *E, const uint8_t *mask,
                            const KWVals<T> &kw, T cs, const Grid &g, const KTiles &kt, const T *r, const T *d, T w,
-                           double *partial, T *xout) {
+                           double *partial, T *xout, const float *mk) {
+    if constexpr (sizeof(T) == 4 && !CK) {
+        if (mk) {  // element products on the matrix cores
+            if (epi == EPI_KU)
+                k_kfine_walsh<T, EPI_KU, false, DOT, false, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            else if (epi == EPI_RESID)
+                k_kfine_walsh<T, EPI_RESID, false, false, false, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            else if (epi == EPI_RESID_JAC0)
+                k_kfine_walsh<T, EPI_RESID_JAC0, false, false, false, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            else
+                k_kfine_walsh<T, EPI_JACOBI, false, DOT, false, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            return;
+        }
+    }
     if (epi == EPI_KU)
         k_kfine_walsh<T, EPI_KU, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
     else if (epi == EPI_RESID)
@@ -499,16 +562,17 @@ static void launch_walsh_k(int epi, unsigned gt, hipStream_t st, const T *u, T *
         k_kfine_walsh<T, EPI_JACOBI, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
 }
 
+// mk (FP32 only, may be null): the element matrix in FP32 for the matrix-core element products
 template <typename T>
 static void launch_walsh(int epi, bool ck, unsigned gt, hipStream_t st, const T *u, T *out, const T *E,
                          const uint8_t *mask, const KWVals<T> &kw, T cs, const Grid &g, const KTiles &kt,
-                         const T *r, const T *d, T w, double *partial, T *xout) {
-    if (ck) {
-        if (partial) launch_walsh_k<T, true, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
-        else launch_walsh_k<T, true, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+                         const T *r, const T *d, T w, double *partial, T *xout, const float *mk = nullptr) {
+    if (ck && !mk) {
+        if (partial) launch_walsh_k<T, true, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
+        else launch_walsh_k<T, true, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
     } else {
-        if (partial) launch_walsh_k<T, false, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
-        else launch_walsh_k<T, false, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout);
+        if (partial) launch_walsh_k<T, false, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, mk);
+        else launch_walsh_k<T, false, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, mk);
     }
 }
 
@@ -549,7 +613,7 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
                 float *o2 = (float *)out;
                 const float w32 = (float)omega;
                 launch_walsh<float>(epi, const_k0, gt, st, uu, o2, E2, D.nodeMask, k, (float)model.prm.cell_size, D.g, kt, r2, d2, w32, dp,
-                                    (float *)xout);
+                                    (float *)xout, kfine_mfma ? K0_32.p : nullptr);
             }
             if (ev) TG_HIP(hipEventRecord(ev[1], st), "tg_solver:kbyu");
             if (dp) {
@@ -667,7 +731,11 @@ void Solver::vcycle_t(const T *r0, T *z0) {
             const unsigned gt = (unsigned)kt.count();
             if ((int64_t)dotp.n < (int64_t)gt) dotp.alloc(gt);
             const float *x32 = (const float *)xx[0], *r32 = (const float *)rr[0], *d32 = D.diag32.p;
-            if (const_k0)
+            if (kfine_mfma)
+                k_kfine_walsh<float, EPI_JACOBI, false, true, true, true><<<gt, 256, 0, st>>>(
+                    x32, (float *)out, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32,
+                    (float)omega, dotp.p, nullptr, r64_dot, K0_32.p);
+            else if (const_k0)
                 k_kfine_walsh<float, EPI_JACOBI, true, true, true><<<gt, 256, 0, st>>>(
                     x32, (float *)out, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32,
                     (float)omega, dotp.p, nullptr, r64_dot);
@@ -970,6 +1038,7 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
         else if (key == TG_TUNE_KELEM_PARTS && (value == 4 || value == 6 || value == 8 || value == 12 || value == 24))
             S.kelem_parts = value;
         else if (key == TG_TUNE_COARSE_GRAPH && (value == 0 || value == 1)) S.coarse.use_graph = value != 0;
+        else if (key == TG_TUNE_KFINE_MFMA && (value == 0 || value == 1)) S.kfine_mfma = value;
         else if (key == TG_TUNE_TIME_KFINE && value >= 0 && value <= 2) {
             S.time_fine = value;
             S.tused = 0;

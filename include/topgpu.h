@@ -83,6 +83,16 @@ int Scattering_inplace(int nlhs, tg_array *plhs, int nrhs, const tg_array *prhs,
  * Replaces scatter_accum3_inplace.cu:59-145 (kernel :32-56). */
 int scatter_accum3_inplace(int nlhs, tg_array *plhs, int nrhs, const tg_array *prhs, void *stream);
 
+/* Scattering_inplace_noalloc: Scattering_inplace (same arguments, checks, error identifiers and
+ * results) that allocates nothing.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * The incidence is sorted on every call in a fixed 128 MiB device array of the library, so inputs are
+ * limited to what fits there (16 x 8 Ne bytes of keys and ids plus the sort's storage: about 1.0M
+ * elements); larger ones fail with Scattering_inplace:shape before any device work.  All calls on one
+ * GPU share that array: calls that overlap on one GPU race and must be serialised by the caller.
+ * Synchronises. */
+int Scattering_inplace_noalloc(int nlhs, tg_array *plhs, int nrhs, const tg_array *prhs, void *stream);
+
 /* assembleKs_level2_inplace(d_Ks[24x24xNe], d_upMap[Ne x nCh] int32, d_eleModulus, d_K_0[576],
  *                           d_Pch[24x24xnCh], d_mapUnique int32, d_uniqueFree[576 x nU] | empty,
  *                           d_uniqueFixed[576 x nU] | empty):

@@ -57,6 +57,7 @@ SIGNATURES = {
     'Gathering_inplace': (ctypes.c_int, [ctypes.c_int, _PA, ctypes.c_int, _PA, _VP]),
     'Scattering_inplace': (ctypes.c_int, [ctypes.c_int, _PA, ctypes.c_int, _PA, _VP]),
     'scatter_accum3_inplace': (ctypes.c_int, [ctypes.c_int, _PA, ctypes.c_int, _PA, _VP]),
+    'Scattering_inplace_noalloc': (ctypes.c_int, [ctypes.c_int, _PA, ctypes.c_int, _PA, _VP]),
     'assembleKs_level2_inplace': (ctypes.c_int, [ctypes.c_int, _PA, ctypes.c_int, _PA, _VP]),
     'assembleKs_higherLevel_inplace': (ctypes.c_int, [ctypes.c_int, _PA, ctypes.c_int, _PA, _VP]),
     'assembleKs_level2_superEle_inplace': (ctypes.c_int, [ctypes.c_int, _PA, ctypes.c_int, _PA, _VP]),

@@ -58,6 +58,11 @@ def Scattering_inplace(Y, eNodMat, uMat, stream=None):
     _call('Scattering_inplace', [Y, eNodMat, uMat], stream=stream)
 
 
+def Scattering_inplace_noalloc(Y, eNodMat, uMat, stream=None):
+    """Scattering_inplace without any device allocation (fixed library scratch; see include/topgpu.h)."""
+    _call('Scattering_inplace_noalloc', [Y, eNodMat, uMat], stream=stream)
+
+
 def scatter_accum3_inplace(keys, Tmp, xFiner, stream=None):
     _call('scatter_accum3_inplace', [keys, Tmp, xFiner], stream=stream)
 

@@ -56,7 +56,9 @@ class Run:
         self.G, self.L, self.torch = G, _lib, torch
         self.V0, self.Emin = V0, Emin
         self.models = {p: cuboid_model(*shape, 1, precision=p) for p in ('fp64', 'mixed')}
+        self.models['bf16'] = cuboid_model(*shape, 1, precision='mixed')
         self.solvers = {p: Solver(self.models[p]) for p in self.models}
+        self.solvers['bf16'].set_tuning(kfine_mfma='bf16')
         self.model = self.models['fp64']
         self.filt = G.PDEFilter(self.model, rMin, tol=1e-4, max_it=800)
         self.Ne = self.model.num_elements
@@ -90,7 +92,7 @@ class Run:
         G, L, torch = self.G, self.L, self.torch
         t0 = time.perf_counter()
         E = G.simp(self.xPhys, self.Emin, 1.0, 3.0)
-        r = {p: self.solve(p, E) for p in ('fp64', 'mixed')}
+        r = {p: self.solve(p, E) for p in ('fp64', 'mixed', 'bf16')}
         t1 = time.perf_counter()
         ce = G.unit_compliance(self.solvers['fp64'], self.U['fp64'])
         dc = G.sensitivity(self.xPhys, ce, self.cSolid, self.Emin, 1.0, 3.0)
@@ -103,7 +105,7 @@ class Run:
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         r['update_s'] = t2 - t1
-        r['iter_s'] = t2 - t0 - r['mixed']['pcg_s'] - r['mixed']['assembly_s']
+        r['iter_s'] = t2 - t0 - sum(r[p]['pcg_s'] + r[p]['assembly_s'] for p in ('mixed', 'bf16'))
         return r
 
 
@@ -158,7 +160,8 @@ def cpu_baseline(shape, E, F, its_ref, sample_iters):
 
 
 def traffic_probe(shape):
-    """Finest K*u launches at `shape` (moduli 1e-6 + U(0,1)^3, seed 0) and two 2 GiB device copies
+    """Finest K*u launches at `shape` (moduli 1e-6 + U(0,1)^3, seed 0; 3 + 3 timed applies and 3 PCG
+    iterations, whose operator is the FP64 product with the fused p.Kp) and two 2 GiB device copies
     (the runtime blit and a plain elementwise load/store kernel): under rocprofv3 --pmc the copies,
     which move exactly 2 GiB in and 2 GiB out, give the factors that turn FETCH_SIZE and WRITE_SIZE
     into HBM bytes."""
@@ -172,6 +175,10 @@ def traffic_probe(shape):
     s.assemble()
     torch.cuda.synchronize()
     log('finest K*u: fp64 %.1f us, fp32 %.1f us' % (s.time_kbyu(1, 'fp64', 3) * 1e3, s.time_kbyu(1, 'fp32', 3) * 1e3))
+    # three PCG iterations: the PCG operator itself (the FP64 product with the fused p.Kp)
+    F = torch.from_numpy(m.get_table(1, 'F')[:, 0].copy()).cuda()
+    s.pcg(F, max_it=3)
+    torch.cuda.synchronize()
     del s, m
     torch.cuda.empty_cache()
     a = torch.empty(256 * 1024 * 1024, dtype=torch.float64, device='cuda')
@@ -285,9 +292,9 @@ def main():
     ap.add_argument('--traffic-probe', action='store_true',
                     help='only run 3 + 3 finest K*u launches (FP64, FP32) and the 2 GiB copies that calibrate '
                          'the FETCH_SIZE / WRITE_SIZE counters; for rocprofv3 --pmc runs')
-    ap.add_argument('--kfine-walsh', action='store_true',
-                    help='mixed solver: FP32 finest element products in the Walsh form on the VALU instead of the '
-                         'matrix cores (for comparison)')
+    ap.add_argument('--kfine', choices=('walsh', 'mfma32'), default='walsh',
+                    help='mixed solver: finest element products in the Walsh form on the VALU (default) or dense on '
+                         'MFMA with FP32 operands (for comparison)')
     ap.add_argument('--mex', type=int, default=0, metavar='REPS',
                     help='only time the six MEX entry points (REPS calls each) at the --size shapes')
     args = ap.parse_args()
@@ -308,8 +315,7 @@ def main():
     x0 = 0.12 + rng.uniform(-0.05, 0.05, Ne_full)
     t = time.perf_counter()
     run = Run(shape, x0)
-    if args.kfine_walsh:
-        run.solvers['mixed'].set_tuning(kfine_mfma=False)
+    run.solvers['mixed'].set_tuning(kfine_mfma=args.kfine)
     torch.cuda.synchronize()
     log('setup + solid solves %.1fs (Ne=%d)' % (time.perf_counter() - t, run.Ne))
     hist = []
@@ -322,19 +328,21 @@ def main():
         r = run.iterate()
         hist.append(r)
         log('it %d: fp64 PCG %.1f ms (%d its) c=%.12e | mixed PCG %.1f ms (%d its) c=%.12e | rel %.1e | '
-            'assembly %.0f ms, FP64 iteration %.0f ms'
+            'bf16 PCG %.1f ms (%d its) rel %.1e | assembly %.0f ms, FP64 iteration %.0f ms'
             % (k + 1, 1e3 * r['fp64']['pcg_s'], r['fp64']['its'], r['fp64']['c'], 1e3 * r['mixed']['pcg_s'],
                r['mixed']['its'], r['mixed']['c'], abs(r['mixed']['c'] - r['fp64']['c']) / abs(r['fp64']['c']),
+               1e3 * r['bf16']['pcg_s'], r['bf16']['its'], abs(r['bf16']['c'] - r['fp64']['c']) / abs(r['fp64']['c']),
                1e3 * r['fp64']['assembly_s'], 1e3 * r['iter_s']))
     timed = hist[args.warmup:]
     res = {}
-    for p in ('fp64', 'mixed'):
+    for p in ('fp64', 'mixed', 'bf16'):
         pcg = sum(h[p]['pcg_s'] for h in timed)
         res[p] = dict(solves_per_s=len(timed) / pcg, ms_per_solve=1e3 * pcg / len(timed),
                       pcg_its=[h[p]['its'] for h in timed], assembly_ms=1e3 * np.mean([h[p]['assembly_s'] for h in timed]),
                       compliance=[h[p]['c'] for h in timed])
     res['fp64']['iteration_ms'] = 1e3 * np.mean([h['iter_s'] for h in timed])
     errs = [abs(h['mixed']['c'] - h['fp64']['c']) / abs(h['fp64']['c']) for h in timed]
+    errs16 = [abs(h['bf16']['c'] - h['fp64']['c']) / abs(h['fp64']['c']) for h in timed]
     # finest K*u: average launch duration over the timed steps (HIP events around every launch on the
     # solver stream); FP64 = the PCG operator of the FP64 solver, FP32 = the mixed V-cycle's residual
     # and smoothing applies
@@ -380,8 +388,12 @@ def main():
         'mixed': res['mixed'],
         'mixed_over_fp64': res['mixed']['solves_per_s'] / res['fp64']['solves_per_s'],
         'rel_compliance_err_mixed_vs_fp64': {'max': max(errs), 'per_iteration': errs},
+        'mixed_bf16': dict(res['bf16'], element_products='dense 24x24 on MFMA f32 32x32x16 bf16 (BF16 operands, '
+                                                         'FP32 accumulation), rest of the mixed V-cycle FP32',
+                           over_fp64=res['bf16']['solves_per_s'] / res['fp64']['solves_per_s'],
+                           rel_compliance_err_vs_fp64={'max': max(errs16), 'per_iteration': errs16}),
         'kbyu': kst,
-        'roofline': {'kernel': 'finest-level K*u, FP64, PCG operator with the fused p.Kp (k_kfine_walsh<double,0,true,true,false>), '
+        'roofline': {'kernel': 'finest-level K*u, FP64, PCG operator with the fused p.Kp (k_kfine_walsh<double,0,true,true,false,0>), '
                                'launches of the mixed solver in the timed steps', 'bound': 'hbm',
                      'achieved': gbs64, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': gbs64 / HBM_PEAK_GBS,
                      'traffic': KFINE64_TRAFFIC, 'bytes_alg': b64, 'avg_launch_us': us64, 'launches': kst['fp64_ku']['launches'],

@@ -234,8 +234,9 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *                         V-cycle's packed symmetric product always uses one thread per element
  *   TG_TUNE_COARSE_GRAPH  1 = replay the coarsest solve's launch sequence as a captured HIP graph,
  *                         0 = launch it kernel by kernel (default: measured 1% faster per PCG iteration)
- *   TG_TUNE_KFINE_MFMA    FP32 finest K*u of the mixed V-cycle: 1 = element products on the matrix
- *                         cores (dense 24 x 24 on MFMA f32 32x32x2, default), 0 = Walsh form on the VALU */
+ *   TG_TUNE_KFINE_MFMA    element products of the mixed V-cycle's finest K*u: 0 = Walsh form on the
+ *                         VALU (default, the fastest measured), 1 = dense 24 x 24 on MFMA f32 32x32x2,
+ *                         2 = dense on MFMA f32 32x32x16 bf16 (BF16 operands, FP32 accumulation) */
 enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4,
        TG_TUNE_COARSE_GRAPH = 5, TG_TUNE_KFINE_MFMA = 6 };
 int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value);

@@ -257,10 +257,11 @@ def test_mixed_vcycle_matches_fp64(super_element):
 
 @pytest.mark.parametrize('nu', [0.3, 0.27])
 def test_mixed_vcycle_matrix_core_element_products(nu):
-    """The FP32 finest K*u with its element products on the matrix cores (dense 24 x 24 on MFMA f32
-    32x32x2, the mixed V-cycle's default) against the Walsh form on the VALU, inside the mixed V-cycle,
-    on a perforated grid spanning several tiles and z chunks (chunk sizes 1, 3, 16): both differ from the
-    FP64 V-cycle by FP32 rounding only, and from each other by less."""
+    """The mixed V-cycle's finest element products on the matrix cores (dense 24 x 24: MFMA f32 32x32x2 with
+    FP32 operands, and MFMA f32 32x32x16 bf16 with BF16 operands and FP32 accumulation) against the Walsh
+    form on the VALU (the default), on a perforated grid spanning several tiles and z chunks (chunk sizes
+    1, 3, 16): the FP32 products differ from the FP64 V-cycle by FP32 rounding only; the BF16 ones by the
+    operands' 8-bit mantissas (element products to ~2^-9)."""
     import torch
     from top_gpu_on_matlab_amd.model import Model
     from top_gpu_on_matlab_amd.solver import Solver
@@ -277,17 +278,19 @@ def test_mixed_vcycle_matrix_core_element_products(nu):
         sv[p] = s
     r = torch.from_numpy(np.random.default_rng(8).standard_normal(sv['fp64'].ndof(1))).cuda()
     z64 = sv['fp64'].vcycle(r)
-    sv['mixed'].set_tuning(kfine_mfma=False)
+    sv['mixed'].set_tuning(kfine_mfma='walsh')
     zw = sv['mixed'].vcycle(r)
-    sv['mixed'].set_tuning(kfine_mfma=True)
-    for zc in (1, 3, 16):
-        sv['mixed'].set_tuning(kfine_zchunk=zc)
-        zm = sv['mixed'].vcycle(r)
-        dm = float((zm - z64).abs().max() / z64.abs().max())
-        dw = float((zw - z64).abs().max() / z64.abs().max())
-        dmw = float((zm - zw).abs().max() / zw.abs().max())
-        print('nu %.2f chunk %d: mfma vs fp64 %.1e, walsh vs fp64 %.1e, mfma vs walsh %.1e' % (nu, zc, dm, dw, dmw),
-              flush=True)
-        assert dm < 2e-5 and dmw < 2e-5, (zc, dm, dw, dmw)
-        assert torch.equal(sv['mixed'].vcycle(r), zm)  # deterministic
+    dw = float((zw - z64).abs().max() / z64.abs().max())
+    for mode, bar in (('mfma32', 2e-5), ('bf16', 2e-2)):
+        sv['mixed'].set_tuning(kfine_mfma=mode)
+        for zc in (1, 3, 16):
+            sv['mixed'].set_tuning(kfine_zchunk=zc)
+            zm = sv['mixed'].vcycle(r)
+            dm = float((zm - z64).abs().max() / z64.abs().max())
+            dmw = float((zm - zw).abs().max() / zw.abs().max())
+            print('nu %.2f %s chunk %d: vs fp64 %.1e (walsh vs fp64 %.1e), vs walsh %.1e' % (nu, mode, zc, dm, dw, dmw),
+                  flush=True)
+            assert dm < bar and dmw < bar, (mode, zc, dm, dw, dmw)
+            assert torch.equal(sv['mixed'].vcycle(r), zm)  # deterministic
+    sv['mixed'].set_tuning(kfine_mfma='walsh', kfine_zchunk=12)
     torch.cuda.synchronize()

@@ -144,20 +144,25 @@ __device__ __forceinline__ void walsh_inv(T x[8]) {
 // R64 (FP32 damped Jacobi with DOT only): the dot is taken against rdot, the FP64 residual, instead of
 // the FP32 one -- the mixed V-cycle's last step, giving the FP64 CG its z.r (z itself stays FP32: its
 // widening to FP64 is exact and happens where the CG reads it).
-// MF (FP32 only): the element product runs on the matrix cores as a dense 24 x 24 product instead of
-// the Walsh form (mfma_K0: the element matrix, cellSize included, column-major FP32; kw and cs are
-// unused).  Wave w takes the layer's elements 64 w .. 64 w + 63 as two blocks of 32; one MFMA f32 32x32x2
-// step multiplies 32 output rows of K0 by 2 input DOFs of the block's 32 elements, 12 steps per block.
-// Input DOF 2 s + h is value s (vertex s / 3 of the face, component s % 3) of face h (h = 0: node plane
-// ez, 1: ez + 1), so lane (n, h) (n = lane % 32, h = lane / 32) reads the 12 values of face h of element n
-// from LDS.  Output rows are permuted so that lane half h holds, in result registers 0..11, the 12 values
-// of face h ^ (ez & 1) of its element; the permutation alternates with the layer parity, so the upper
-// face a lane computes in one layer and the lower face of the element above it in the next layer fall
-// to the same lane and are summed in registers, as the Walsh form does.  Lane layouts of the
-// instruction: lane l holds A(l % 32, l / 32) and B(l / 32, l % 32); result register 4 g + i is at row
-// 8 g + 4 (l / 32) + i, column l % 32.
-template <typename T, int EPI, bool CK, bool DOT, bool R64 = false, bool MF = false>
-__global__ __launch_bounds__(256, (sizeof(T) == 8 || MF) ? 3 : 4) void k_kfine_walsh(const T *__restrict__ u, T *__restrict__ out,
+// MF (FP32 only; 0 = the Walsh form): the element product runs on the matrix cores as a dense 24 x 24
+// product (mfma_K0: the element matrix, cellSize included, column-major FP32; kw and cs are unused).
+// MF = 1: MFMA f32 32x32x2, FP32 operands.  MF = 2: MFMA f32 32x32x16 bf16, operands (E_e u_e and K0)
+// rounded to BF16, products accumulated in FP32.  Wave w takes the layer's elements 64 w .. 64 w + 63 as
+// two blocks of 32 (columns of the product).  Lane (n, h) (n = lane % 32, h = lane / 32) supplies the 12
+// values of face h (h = 0: node plane ez, 1: ez + 1) of element n, read from LDS: with MF = 1 as input
+// DOFs 2 s + h (value s = vertex s / 3 of the face, component s % 3), 12 steps per block; with MF = 2 as
+// input DOFs 16 s + 8 h + j (value 8 s + j, j < 8; values past 11 are zero padding), 2 steps per block.
+// Output rows are permuted so that lane half h holds, in result registers 0..11, the 12 values of face
+// h ^ (ez & 1) of its element; the permutation alternates with the layer parity, so the upper face a
+// lane computes in one layer and the lower face of the element above it in the next layer fall to the
+// same lane and are summed in registers, as the Walsh form does.  Lane layouts: lane l holds A(l % 32,
+// K-block l / 32) and B(K-block l / 32, l % 32) (one value per lane for f32 32x32x2, eight consecutive K
+// for bf16 32x32x16); result register 4 g + i is at row 8 g + 4 (l / 32) + i, column l % 32.
+// Measured at 128 x 256 x 128 the dense products are slower than the Walsh form (MF = 1: 141 / 174 us
+// against 94 / 89 us for the residual / smoothing applies): 1536 padded flops per element on the FP32
+// matrix pipe take >= 58 us alone, where the Walsh form spends ~250 VALU operations per element.
+template <typename T, int EPI, bool CK, bool DOT, bool R64 = false, int MF = 0>
+__global__ __launch_bounds__(256, (sizeof(T) == 8 || MF != 0) ? 3 : 4) void k_kfine_walsh(const T *__restrict__ u, T *__restrict__ out,
                                                      const T *__restrict__ E, const uint8_t *__restrict__ mask,
                                                      const KWVals<T> kw, T cs, const Grid g, const KTiles kt,
                                                      const T *__restrict__ r, const T *__restrict__ diag, T omega,
@@ -165,7 +170,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 || MF) ? 3 : 4) void k_kfine_w
                                                      const double *__restrict__ rdot,
                                                      const float *__restrict__ mfma_K0 = nullptr) {
     static_assert(!R64 || (EPI == EPI_JACOBI && DOT && sizeof(T) == 4), "R64: FP32 Jacobi with the dot only");
-    static_assert(!MF || sizeof(T) == 4, "MF: FP32 only");
+    static_assert(MF == 0 || sizeof(T) == 4, "MF: FP32 only");
     constexpr bool J0 = EPI == EPI_RESID_JAC0;
     __shared__ T sU[2][3][KT_PN];
     __shared__ uint8_t sM[2][KT_PN];
@@ -247,27 +252,20 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 || MF) ? 3 : 4) void k_kfine_w
         Eok = eIn && (unsigned)layer < (unsigned)g.EZ;
     };
     // the owned node's own values for the epilogue (raw u: identity rows and the dot; r; diag),
-    // loaded one plane ahead, after this plane's stores.  u is already staged in LDS (slot `slot` holds
-    // plane z and stays untouched until every lane has passed the next layer's barriers), except that
-    // fixed DOFs were zeroed there: only those are read from memory again.  That load stays
-    // unconditional (lanes without a fixed DOF read u[0], one cached line), so the loop keeps its static
-    // memory-counter waits.
+    // loaded one plane ahead, after this plane's stores.  u is read from memory again rather than from
+    // the LDS stage: the stage has fixed DOFs zeroed, and patching them in costs the FP64 operator more
+    // (a select on a fresh load waits for it, draining the next plane's prefetch; measured 104 against
+    // 95 us at 128 x 256 x 128) than the 12 or 24 bytes per node the reload reads.
     constexpr bool NEED_U = EPI == EPI_KU || EPI == EPI_RESID || EPI == EPI_JACOBI;
     constexpr bool NEED_R = EPI != EPI_KU;
     constexpr bool NEED_D = EPI == EPI_JACOBI || J0;
     T oU[3] = {0, 0, 0}, oR[3] = {0, 0, 0}, oD[3] = {1, 1, 1};
     double oW[3] = {0, 0, 0};  // R64: the FP64 residual for the dot
-    const int posO = (oy + 1) + KT_PY * (ox + 1);  // the owned node in the staged plane (owners only)
-    auto fetchO = [&](int z, int slot) {
+    auto fetchO = [&](int z) {
         const int64_t b = 3 * NYX * zclamp(z, g.NZ) + offO;
-        const uint8_t mkn = (NEED_U && owner && slot >= 0) ? sM[slot][posO] : (uint8_t)0;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            if (NEED_U) {
-                const bool fx = (mkn & (1 << d)) != 0;
-                const T raw = u[fx ? b + d : 0];
-                oU[d] = fx ? raw : ((owner && slot >= 0) ? sU[slot][d][posO] : (T)0);
-            }
+            if (NEED_U) oU[d] = u[b + d];
             if (NEED_R) oR[d] = r[b + d];
             if (NEED_D) oD[d] = diag[b + d];
             if (R64) oW[d] = rdot[b + d];
@@ -278,35 +276,43 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 || MF) ? 3 : 4) void k_kfine_w
     commit(0);
     fetch(z0);
     fetchE(z0 - 1);
-    fetchO(z0 - 1, -1);  // plane z0 - 1 is never written
+    fetchO(z0 - 1);
     // the upper face of this lane's element (the face of the same (y, x) in the next node plane), in
     // the face's Walsh basis [3 i + c]: completed by the next layer's element before it is inverted
     T kup[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     // FP32 (register room for it): the forward y and x butterflies of the element's lower face are
     // those its upper face had in the previous layer, kept in klo [3 q + c]; each face is transformed
     // once.  FP64 transforms both faces of every element (12 more registers would cost a wave).
-    constexpr bool KEEPF = sizeof(T) == 4 && !MF;
+    constexpr bool KEEPF = sizeof(T) == 4 && MF == 0;
     T klo[KEEPF ? 12 : 1];
     // MF: the lane's A operand for both layer parities, and the face kept from the previous layer
+    typedef __bf16 mf_bf8 __attribute__((ext_vector_type(8)));
     const int ml = t & 63, mn = ml & 31, mh = ml >> 5, mw = t >> 6;
-    float mA[MF ? 2 : 1][MF ? 12 : 1];
+    float mA[MF == 1 ? 2 : 1][MF == 1 ? 12 : 1];
+    mf_bf8 mAb[MF == 2 ? 2 : 1][MF == 2 ? 2 : 1];
     float mkept[MF ? 2 : 1][MF ? 12 : 1];
-    if constexpr (MF) {
+    if constexpr (MF != 0) {
         const int g8 = mn >> 3, hh = (mn >> 2) & 1, ii = mn & 3, j = 4 * g8 + ii;  // output row mn
+        // entry of the permuted K0: output row mn at parity P, input value v (0..11) of face mh
+        auto aval = [&](int P, int v) -> float {
+            if (j >= 12 || v >= 12) return 0.0f;
+            const int qi = v / 3, ci = v % 3, qo = j / 3, co = j % 3;
+            const int col = 3 * vtx_of(qi & 1, qi >> 1, mh) + ci;
+            const int row = 3 * vtx_of(qo & 1, qo >> 1, hh ^ P) + co;
+            return mfma_K0[row + 24 * col];
+        };
 #pragma unroll
-        for (int P = 0; P < 2; ++P)
+        for (int P = 0; P < 2; ++P) {
+            if constexpr (MF == 1) {
 #pragma unroll
-            for (int s2 = 0; s2 < 12; ++s2) {
-                const int qi = s2 / 3, ci = s2 % 3;  // input DOF 2 s2 + mh: face mh, value s2
-                const int col = 3 * vtx_of(qi & 1, qi >> 1, mh) + ci;
-                float v = 0.0f;
-                if (j < 12) {
-                    const int qo = j / 3, co = j % 3;
-                    const int row = 3 * vtx_of(qo & 1, qo >> 1, hh ^ P) + co;
-                    v = mfma_K0[row + 24 * col];
-                }
-                mA[P][s2] = v;
+                for (int s2 = 0; s2 < 12; ++s2) mA[P][s2] = aval(P, s2);
+            } else {
+#pragma unroll
+                for (int s2 = 0; s2 < 2; ++s2)
+#pragma unroll
+                    for (int jj = 0; jj < 8; ++jj) mAb[P][s2][jj] = (__bf16)aval(P, 8 * s2 + jj);
             }
+        }
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -342,7 +348,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 || MF) ? 3 : 4) void k_kfine_w
         fetch(ez + 2);  // past the chunk on the last layer: clamped, never committed
         fetchE(ez + 1);
         // ---- element vector of this lane's element, passed on as face values
-        if constexpr (MF) {
+        if constexpr (MF != 0) {
             typedef float mf_f16 __attribute__((ext_vector_type(16)));
             const int P = ez & 1;
             // E of this lane's two elements: its own (block mh) and its partner lane's (block 1 - mh)
@@ -361,9 +367,20 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 || MF) ? 3 : 4) void k_kfine_w
                 }
 #pragma unroll
                 for (int k = 0; k < 16; ++k) acc[b][k] = 0.0f;
+                if constexpr (MF == 1) {
 #pragma unroll
-                for (int s2 = 0; s2 < 12; ++s2)
-                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(P ? mA[1][s2] : mA[0][s2], bv[s2], acc[b], 0, 0, 0);
+                    for (int s2 = 0; s2 < 12; ++s2)
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(P ? mA[1][s2] : mA[0][s2], bv[s2], acc[b], 0, 0, 0);
+                } else {
+                    mf_bf8 b0, b1;
+#pragma unroll
+                    for (int jj = 0; jj < 8; ++jj) {
+                        b0[jj] = (__bf16)bv[jj];
+                        b1[jj] = (__bf16)(jj < 4 ? bv[8 + jj] : 0.0f);
+                    }
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P ? mAb[1][0] : mAb[0][0], b0, acc[b], 0, 0, 0);
+                    acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P ? mAb[1][1] : mAb[0][1], b1, acc[b], 0, 0, 0);
+                }
             }
             // lanes holding the lower face (mh == P) add the face kept from the layer below and pass it on
 #pragma unroll
@@ -486,7 +503,7 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 || MF) ? 3 : 4) void k_kfine_w
                 }
             }
         }
-        fetchO(ez + 1, cur ^ 1);
+        fetchO(ez + 1);
         cur ^= 1;
     }
     if (DOT) {

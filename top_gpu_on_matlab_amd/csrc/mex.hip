@@ -6,7 +6,9 @@
 //   * scatter / scatter_accum3: a stable radix sort of (target row, entry) pairs, kept per
 //     connectivity as a CSR incidence, then one lane per target row sums its entries in ascending
 //     entry order (no floating-point atomics);
-//   * Galerkin assembly: one wavefront per coarse element (galerkin.h).
+//   * Galerkin assembly: one wavefront per coarse element (galerkin.h); the FP64 level-2 entry uses the
+//     product's linearity in the children moduli (a GEMM on the matrix cores) and recomputes only the
+//     children that carry boundary matrices.
 #include <memory>
 #include <mutex>
 
@@ -293,6 +295,129 @@ __global__ __launch_bounds__(GK_THREADS) void k_galerkin_mex(T *__restrict__ Ks,
     for (int i = 0; i < 9; ++i) Ks[(size_t)e * 576 + galerkin_entry(i)] = acc[i];
 }
 
+// ---- level-2 Galerkin through its linearity (FP64) ---------------------------------------------------
+// Ks_e = sum_s Pch_s' Kch_s Pch_s with Kch_s = E_f K0 for children without boundary matrices is linear in
+// the children moduli: Ks_e = sum_s E_s G_s, G_s = Pch_s' K0 Pch_s.  G is formed per call from the
+// caller's Pch and K0, the sum is one GEMM on the matrix cores, and the children with boundary matrices
+// (mapUnique > 0) are added exactly, child by child (galerkin_block).
+
+// G[s * 576 + r + 24 c] = (Pch_s' K0 Pch_s)(r, c), one workgroup per child
+__global__ __launch_bounds__(256) void k_mex_gmat(const double *__restrict__ P, const double *__restrict__ K0,
+                                                  double *__restrict__ G) {
+    __shared__ double sP[576], sK[576], sT[576];
+    const int s = blockIdx.x, t = threadIdx.x;
+    for (int i = t; i < 576; i += 256) {
+        sP[i] = P[(size_t)s * 576 + i];
+        sK[i] = K0[i];
+    }
+    __syncthreads();
+    for (int i = t; i < 576; i += 256) {  // T = K0 P_s
+        const int a = i % 24, c = i / 24;
+        double v = 0.0;
+        for (int b = 0; b < 24; ++b) v += sK[a + 24 * b] * sP[b + 24 * c];
+        sT[i] = v;
+    }
+    __syncthreads();
+    for (int i = t; i < 576; i += 256) {  // G = P_s' T
+        const int r = i % 24, c = i / 24;
+        double v = 0.0;
+        for (int a = 0; a < 24; ++a) v += sP[a + 24 * r] * sT[a + 24 * c];
+        G[(size_t)s * 576 + i] = v;
+    }
+}
+
+// Ech[e + Ne s] = modulus of child s of coarse element e, 0 for void children and children with boundary
+// matrices (those are added by k_mex_galerkin_bnd)
+__global__ void k_mex_ech(const int32_t *__restrict__ up, const double *__restrict__ E, const int32_t *__restrict__ mapU,
+                          int hasFixed, int64_t nFine, int64_t nU, int64_t Ne, int nCh, double *__restrict__ Ech) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Ne * nCh) return;
+    const int32_t f1 = up[t];
+    double v = 0.0;
+    if (f1 > 0 && f1 <= nFine) {
+        const int32_t u1 = hasFixed ? mapU[f1 - 1] : 0;
+        if (!(u1 > 0 && u1 <= nU)) v = E[f1 - 1];
+    }
+    Ech[t] = v;
+}
+
+// Ks[e * 576 + i] = sum_k Ech[e + Ne k] G[k * 576 + i] on MFMA f64 16x16x4 (lane l holds A(l % 16, l / 16),
+// B(l / 16, l % 16), result register r at (4 r + l / 16, l % 16)): A = 16 elements x 4 children of Ech,
+// B = 4 children x 16 entries of G, so every store row is 16 consecutive entries of one element's page.
+// A wavefront owns 16 NG elements; elements past Ne read zero moduli and store nothing.
+typedef double tg_mdbl4 __attribute__((ext_vector_type(4)));
+template <int KS, int NG>
+__global__ __launch_bounds__(256) void k_mex_galerkin_gemm(const double *__restrict__ Ech, const double *__restrict__ G,
+                                                           double *__restrict__ Ks, int64_t Ne) {
+    const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+    const int64_t e0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * NG);
+    if (e0 >= Ne) return;
+    double a[NG][KS];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int64_t e = e0 + 16 * g + li;
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) a[g][kk] = e < Ne ? Ech[e + Ne * (lk + 4 * kk)] : 0.0;
+    }
+    const double *gp = G + (int64_t)lk * 576 + li;
+    for (int t = 0; t < 36; ++t) {
+        tg_mdbl4 acc[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) acc[g] = tg_mdbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            const double b = gp[(int64_t)4 * kk * 576 + 16 * t];
+#pragma unroll
+            for (int g = 0; g < NG; ++g) acc[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[g][kk], b, acc[g], 0, 0, 0);
+        }
+#pragma unroll
+        for (int g = 0; g < NG; ++g)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t e = e0 + 16 * g + 4 * r + lk;
+                if (e < Ne) Ks[e * 576 + 16 * t + li] = acc[g][r];
+            }
+    }
+}
+
+// Children with boundary matrices only (child() skips the others), added to the GEMM's result.
+struct SrcLevel2Bnd : SrcLevel2<double> {
+    __device__ int child(int s) const {
+        const int f = SrcLevel2<double>::child(s);
+        if (f < 0 || !hasFixed) return -1;
+        const int32_t u1 = mapU[f];
+        return (u1 > 0 && u1 <= nU) ? f : -1;
+    }
+};
+
+__global__ __launch_bounds__(GK_THREADS) void k_mex_galerkin_bnd(double *__restrict__ Ks, const int32_t *__restrict__ up,
+                                                                 int64_t Ne, int nCh, const double *__restrict__ P,
+                                                                 const double *E, const double *K0, const double *uf,
+                                                                 const double *ux, const int32_t *mapU, int64_t nFine,
+                                                                 int64_t nU) {
+    __shared__ double sB[576], sK[576], sT[576];
+    const int e = blockIdx.x;
+    SrcLevel2Bnd src;
+    src.up = up;
+    src.Ne = Ne;
+    src.e = e;
+    src.E = E;
+    src.K0 = K0;
+    src.uf = uf;
+    src.ux = ux;
+    src.mapU = mapU;
+    src.hasFixed = true;
+    src.nFine = nFine;
+    src.nU = nU;
+    bool any = false;  // uniform over the wavefront
+    for (int s = 0; s < nCh && !any; ++s) any = src.child(s) >= 0;
+    if (!any) return;
+    double acc[9];
+    galerkin_block(acc, src, nCh, P, sB, sK, sT);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ks[(size_t)e * 576 + galerkin_entry(i)] += acc[i];
+}
+
 }  // namespace
 }  // namespace tg
 
@@ -409,7 +534,21 @@ int assembleKs_level2_inplace(int nlhs, tg_array *plhs, int nrhs, const tg_array
         const int64_t nFine = hasFixed ? std::min(em.numel, mu.numel) : em.numel;
         hipStream_t st = S(stream);
         if (Ne > 0) {
-            if (cls == TG_DOUBLE)
+            if (cls == TG_DOUBLE && (nCh == 64 || nCh == 8)) {
+                DevBuf<double> G((size_t)nCh * 576), Ech((size_t)Ne * nCh);
+                k_mex_gmat<<<nCh, 256, 0, st>>>((const double *)pch.p, (const double *)k0.p, G);
+                k_mex_ech<<<grid_for(Ne * nCh, 256), 256, 0, st>>>((const int32_t *)up.p, (const double *)em.p,
+                                                                  (const int32_t *)mu.p, hasFixed, nFine, nU, Ne, nCh, Ech);
+                const unsigned gg = grid_for(Ne, 256);  // 64 elements per wavefront
+                if (nCh == 64) k_mex_galerkin_gemm<16, 4><<<gg, 256, 0, st>>>(Ech, G, (double *)Ks.p, Ne);
+                else k_mex_galerkin_gemm<2, 4><<<gg, 256, 0, st>>>(Ech, G, (double *)Ks.p, Ne);
+                if (hasFixed)
+                    k_mex_galerkin_bnd<<<(unsigned)Ne, GK_THREADS, 0, st>>>((double *)Ks.p, (const int32_t *)up.p, Ne, nCh,
+                        (const double *)pch.p, (const double *)em.p, (const double *)k0.p, (const double *)uf.p,
+                        (const double *)ux.p, (const int32_t *)mu.p, nFine, nU);
+                TG_LAUNCH_CHECK(Cu);
+                TG_HIP(hipStreamSynchronize(st), Cu);  // G and Ech are released on leaving the scope
+            } else if (cls == TG_DOUBLE)
                 k_galerkin_mex<double, 0><<<(unsigned)Ne, GK_THREADS, 0, st>>>((double *)Ks.p, (const int32_t *)up.p, Ne, nCh, (const double *)pch.p,
                     (const double *)em.p, (const double *)k0.p, (const double *)uf.p, (const double *)ux.p, (const int32_t *)mu.p, hasFixed, nFine, nU);
             else

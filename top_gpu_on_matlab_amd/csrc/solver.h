@@ -49,7 +49,7 @@ struct Solver {
     size_t tused = 0;
     hipEvent_t *timing_slot(int key);
     int kfine_zchunk = 12;  // node planes per workgroup of the tiled kernel
-    int kfine_mfma = 1;     // FP32 tiled finest K*u: element products on the matrix cores (0: Walsh form)
+    int kfine_mfma = 0;     // FP32 tiled finest K*u: 0 Walsh form, 1 / 2 element products on MFMA (FP32 / BF16)
     int kelem_parts = 24;   // threads per element in the per-element-matrix K*u
     bool has_modulus = false, assembled = false;
     std::vector<DevLevel> lv;

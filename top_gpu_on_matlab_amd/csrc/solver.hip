@@ -538,17 +538,28 @@ static void launch_kelem_vec(int parts, hipStream_t st, const T *u, T *fv, const
 template <typename T, bool CK, bool DOT>
 static void launch_walsh_k(int epi, unsigned gt, hipStream_t st, const T *u, T *out, const T *E, const uint8_t *mask,
                            const KWVals<T> &kw, T cs, const Grid &g, const KTiles &kt, const T *r, const T *d, T w,
-                           double *partial, T *xout, const float *mk) {
+                           double *partial, T *xout, const float *mk, int mf) {
     if constexpr (sizeof(T) == 4 && !CK) {
-        if (mk) {  // element products on the matrix cores
+        if (mk && mf == 1) {  // element products on the matrix cores, FP32
             if (epi == EPI_KU)
-                k_kfine_walsh<T, EPI_KU, false, DOT, false, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+                k_kfine_walsh<T, EPI_KU, false, DOT, false, 1><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
             else if (epi == EPI_RESID)
-                k_kfine_walsh<T, EPI_RESID, false, false, false, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+                k_kfine_walsh<T, EPI_RESID, false, false, false, 1><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
             else if (epi == EPI_RESID_JAC0)
-                k_kfine_walsh<T, EPI_RESID_JAC0, false, false, false, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+                k_kfine_walsh<T, EPI_RESID_JAC0, false, false, false, 1><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
             else
-                k_kfine_walsh<T, EPI_JACOBI, false, DOT, false, true><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+                k_kfine_walsh<T, EPI_JACOBI, false, DOT, false, 1><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            return;
+        }
+        if (mk && mf == 2) {  // element products on the matrix cores, BF16 operands, FP32 accumulation
+            if (epi == EPI_KU)
+                k_kfine_walsh<T, EPI_KU, false, DOT, false, 2><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            else if (epi == EPI_RESID)
+                k_kfine_walsh<T, EPI_RESID, false, false, false, 2><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            else if (epi == EPI_RESID_JAC0)
+                k_kfine_walsh<T, EPI_RESID_JAC0, false, false, false, 2><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            else
+                k_kfine_walsh<T, EPI_JACOBI, false, DOT, false, 2><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
             return;
         }
     }
@@ -562,17 +573,19 @@ static void launch_walsh_k(int epi, unsigned gt, hipStream_t st, const T *u, T *
         k_kfine_walsh<T, EPI_JACOBI, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
 }
 
-// mk (FP32 only, may be null): the element matrix in FP32 for the matrix-core element products
+// mk, mf (FP32 only): the element matrix in FP32 and the matrix-core mode (1 FP32, 2 BF16 operands) of the
+// element products; mf = 0 (or mk null): the Walsh form
 template <typename T>
 static void launch_walsh(int epi, bool ck, unsigned gt, hipStream_t st, const T *u, T *out, const T *E,
                          const uint8_t *mask, const KWVals<T> &kw, T cs, const Grid &g, const KTiles &kt,
-                         const T *r, const T *d, T w, double *partial, T *xout, const float *mk = nullptr) {
+                         const T *r, const T *d, T w, double *partial, T *xout, const float *mk = nullptr, int mf = 0) {
+    if (!mf) mk = nullptr;
     if (ck && !mk) {
-        if (partial) launch_walsh_k<T, true, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
-        else launch_walsh_k<T, true, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
+        if (partial) launch_walsh_k<T, true, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, 0);
+        else launch_walsh_k<T, true, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, 0);
     } else {
-        if (partial) launch_walsh_k<T, false, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, mk);
-        else launch_walsh_k<T, false, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, mk);
+        if (partial) launch_walsh_k<T, false, true>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, mk, mf);
+        else launch_walsh_k<T, false, false>(epi, gt, st, u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, mk, mf);
     }
 }
 
@@ -613,7 +626,7 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
                 float *o2 = (float *)out;
                 const float w32 = (float)omega;
                 launch_walsh<float>(epi, const_k0, gt, st, uu, o2, E2, D.nodeMask, k, (float)model.prm.cell_size, D.g, kt, r2, d2, w32, dp,
-                                    (float *)xout, kfine_mfma ? K0_32.p : nullptr);
+                                    (float *)xout, K0_32.p, kfine_mfma);
             }
             if (ev) TG_HIP(hipEventRecord(ev[1], st), "tg_solver:kbyu");
             if (dp) {
@@ -731,8 +744,12 @@ void Solver::vcycle_t(const T *r0, T *z0) {
             const unsigned gt = (unsigned)kt.count();
             if ((int64_t)dotp.n < (int64_t)gt) dotp.alloc(gt);
             const float *x32 = (const float *)xx[0], *r32 = (const float *)rr[0], *d32 = D.diag32.p;
-            if (kfine_mfma)
-                k_kfine_walsh<float, EPI_JACOBI, false, true, true, true><<<gt, 256, 0, st>>>(
+            if (kfine_mfma == 1)
+                k_kfine_walsh<float, EPI_JACOBI, false, true, true, 1><<<gt, 256, 0, st>>>(
+                    x32, (float *)out, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32,
+                    (float)omega, dotp.p, nullptr, r64_dot, K0_32.p);
+            else if (kfine_mfma == 2)
+                k_kfine_walsh<float, EPI_JACOBI, false, true, true, 2><<<gt, 256, 0, st>>>(
                     x32, (float *)out, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32,
                     (float)omega, dotp.p, nullptr, r64_dot, K0_32.p);
             else if (const_k0)
@@ -1038,7 +1055,7 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
         else if (key == TG_TUNE_KELEM_PARTS && (value == 4 || value == 6 || value == 8 || value == 12 || value == 24))
             S.kelem_parts = value;
         else if (key == TG_TUNE_COARSE_GRAPH && (value == 0 || value == 1)) S.coarse.use_graph = value != 0;
-        else if (key == TG_TUNE_KFINE_MFMA && (value == 0 || value == 1)) S.kfine_mfma = value;
+        else if (key == TG_TUNE_KFINE_MFMA && value >= 0 && value <= 2) S.kfine_mfma = value;
         else if (key == TG_TUNE_TIME_KFINE && value >= 0 && value <= 2) {
             S.time_fine = value;
             S.tused = 0;

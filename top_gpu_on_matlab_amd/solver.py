@@ -124,12 +124,14 @@ class Solver:
     def set_tuning(self, kfine_kernel=None, kfine_zchunk=None, kelem_parts=None, coarse_graph=None, kfine_mfma=None):
         """Finest-level K*u launch choice: kernel 'tiled' (default) or 'gather'; z planes per tile;
         threads per element of the FP64 per-element-matrix K*u; coarsest solve replayed as a captured HIP
-        graph (True) or launched kernel by kernel (False, default); FP32 finest element products on the
-        matrix cores (True, default) or in the Walsh form (False)."""
+        graph (True) or launched kernel by kernel (False, default); the mixed V-cycle's finest element
+        products: 'walsh' (the Walsh form on the VALU, default), 'mfma32' (dense, MFMA with FP32 operands)
+        or 'bf16' (dense, MFMA with BF16 operands and FP32 accumulation)."""
         from ._lib import (TG_TUNE_COARSE_GRAPH, TG_TUNE_KELEM_PARTS, TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_MFMA,
                            TG_TUNE_KFINE_ZCHUNK)
         if kfine_mfma is not None:
-            check(lib().tg_solver_set_tuning(self.h, TG_TUNE_KFINE_MFMA, int(bool(kfine_mfma))))
+            mode = {'walsh': 0, 'mfma32': 1, 'bf16': 2}[kfine_mfma] if isinstance(kfine_mfma, str) else int(kfine_mfma)
+            check(lib().tg_solver_set_tuning(self.h, TG_TUNE_KFINE_MFMA, mode))
         if coarse_graph is not None:
             check(lib().tg_solver_set_tuning(self.h, TG_TUNE_COARSE_GRAPH, int(bool(coarse_graph))))
         if kelem_parts is not None:

@@ -28,10 +28,26 @@ import numpy as np
 METRIC = 'PCG solves/sec + K·u HBM GB/s, mixed vs FP64 at 1e-6 rel compliance err'
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per launch of the FP64 finest K*u at 128x256x128 (bytes), from two rocprofv3 --pmc runs of
-# `bench.py --traffic-probe`: FETCH_SIZE 183,906 KiB x 2.0 + WRITE_SIZE 106,650 KiB x 1.0, the factors
-# being what the counters read for a plain 2 GiB device copy (FETCH 1,048,576 KiB, WRITE 2,097,152 KiB)
-KFINE64_TRAFFIC = 485.8e6
+# measured HBM bytes per launch of the FP64 PCG operator: the copy-calibrated FETCH_SIZE + WRITE_SIZE
+# of two rocprofv3 --pmc runs of `bench.py --traffic-probe`, kept with their commands and raw counts in
+# this summary, one `traffic_bytes_per_launch <workload> <kernel> <bytes>` line per measured workload
+TRAFFIC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                               'kfine_walsh_fp64_c5_128x256x128_pmc_traffic.md')
+
+
+def measured_traffic(shape, kernel='fp64_pcg_operator'):
+    """HBM bytes per launch of `kernel` at `shape` from the committed counter summary, or None when that
+    workload has not been measured."""
+    want = '%dx%dx%d' % tuple(shape)
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 4 and w[0] == 'traffic_bytes_per_launch' and w[1] == want and w[2] == kernel:
+                    return float(w[3])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -396,7 +412,7 @@ def main():
         'roofline': {'kernel': 'finest-level K*u, FP64, PCG operator with the fused p.Kp (k_kfine_walsh<double,0,true,true,false,0>), '
                                'launches of the mixed solver in the timed steps', 'bound': 'hbm',
                      'achieved': gbs64, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': gbs64 / HBM_PEAK_GBS,
-                     'traffic': KFINE64_TRAFFIC, 'bytes_alg': b64, 'avg_launch_us': us64, 'launches': kst['fp64_ku']['launches'],
+                     'traffic': measured_traffic(shape), 'bytes_alg': b64, 'avg_launch_us': us64, 'launches': kst['fp64_ku']['launches'],
                      'dense_fp64_flop_frac': tf64 / FP64_PEAK_TFLOPS},
         'cpu_baseline': cpu,
     }

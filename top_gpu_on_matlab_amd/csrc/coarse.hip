@@ -457,26 +457,30 @@ void launch_chain(const ChainProb &A, const ChainProb &B, hipStream_t s) {
 }
 
 // ---- the solve as one persistent launch -------------------------------------------------------------------
-// Launched step by step, the solve's 2 P - 1 dependent steps each pay a launch gap beside a few us of
-// work.  Here every step is a stage of one launch whose workgroups stay resident and meet at a grid
-// barrier between stages: an arrival counter that only grows, so stage s of a launch with base b waits
-// for b + (s + 1) G arrivals (G workgroups; the host advances b by (stages - 1) G per launch).  A stage
-// holds one or two row problems  y = c + M1 v1 - M2 [v2a; v2b]  (one wavefront per row; the two chains
-// of a step split the workgroups in proportion to their rows).  The forward stages fold the
-// block-diagonal product into the recurrence, z_p = S_p^-1 b_p - F_p z_{p-1}, so nothing precedes them.
-// The recurrences' vectors cross workgroups inside the launch, so they are stored and loaded with
-// device-scope atomic accesses, coherent across the XCDs' separate L2 caches, and every wave waits for
-// its stores to complete before its workgroup arrives.  Before arriving, each wave loads the first
-// matrix chunk of its row in the next stage (the matrices do not depend on the barrier), so that load
-// overlaps the wait.  A wait longer than a bounded number of polls sets an error flag and lets the
-// workgroup go on, so the launch always ends; the host checks the flag.  G is at most the number of
-// workgroups the device holds at once, so every workgroup is resident while others wait.
+// Launched step by step, the solve's 2 P - 1 dependent steps each start from cold: a step's matrix rows
+// are fetched only once its launch runs.  Here every step is a stage of one launch whose workgroups
+// stay resident and pass the recurrence vectors to each other directly.  A stage holds one or two row
+// problems  y = c + M1 v1 - M2 [v2a; v2b],  one wavefront per row (the two chains of a step split the
+// workgroups in proportion to their rows).  The forward stages fold the block-diagonal product into
+// the recurrence, z_p = S_p^-1 b_p - F_p z_{p-1}, and write zz; the separator and the backward stages,
+// x_p = z_p - B_p x_{p+1}, write xx.
+// Dataflow instead of barriers: before the launch zz and xx are filled with a sentinel bit pattern (a
+// signalling NaN that arithmetic never produces), every entry is written once per launch, and a
+// workgroup that needs an earlier stage's vector polls exactly the entries it stages until they are no
+// longer the sentinel.  Stores and these loads are device-scope atomic accesses, coherent across the
+// XCDs' separate L2 caches.  (A grid barrier between stages measured 2.6 us with per-group counters and
+// 8 us with one counter, on top of the exchange itself.)  Each workgroup runs ahead as far as its inputs
+// allow: the part of a stage that does not depend on earlier stages (staging the right-hand-side plane,
+// the M1 v1 partial sums, the first chunk of the M2 row) is done before it polls.  A poll that waits
+// longer than a bounded number of tries sets an error flag and gives up, so the launch always ends;
+// the host checks the flag.  G is at most the number of workgroups the device holds at once, so every
+// producer is resident while consumers wait.
 struct PRow {
     const double *M1, *M2;  // row-major, rows zero-padded to ld1 / ld2 (nullptr: no such term)
     int64_t ld1, ld2;
     const double *v1;         // n1 entries (a plane of the right-hand side, written before the launch)
-    const double *v2a, *v2b;  // n2a, n2b entries (recurrence vectors of an earlier stage)
-    const double *c;          // nullptr: none; else y[r] = c[r] + ... (c may alias y)
+    const double *v2a, *v2b;  // n2a, n2b entries (written by earlier stages of the launch)
+    const double *c;          // nullptr: none; else y[r] = c[r] + ... (written by an earlier stage)
     double *y;
     int n, n1, n2a, n2b;
 };
@@ -486,112 +490,133 @@ struct PStage {
     int gA;  // workgroups of q[0] when nq == 2
 };
 
+constexpr unsigned long long PERS_EMPTY = 0x7ff4c0a5e5e17e11ull;  // signalling NaN: "not written yet"
+constexpr unsigned PERS_POLLS = 1u << 22;                         // ~0.1 s of polling before giving up
+
 // The table's pointers are generic; these name them as global memory, so that the loads are global
 // (not flat) accesses.
 typedef __attribute__((address_space(1))) const double gcdouble;
 typedef __attribute__((address_space(1))) unsigned long long gull;
 __device__ __forceinline__ gcdouble *gmem(const double *p) { return (gcdouble *)p; }
-__device__ __forceinline__ double coh_load(const double *p) {
-    return __longlong_as_double((long long)__hip_atomic_load((gull *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+__device__ __forceinline__ unsigned long long coh_bits(const double *p) {
+    return __hip_atomic_load((gull *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void coh_store(double *p, double v) {
     __hip_atomic_store((gull *)p, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
 }
+// The value at p once an earlier stage has written it (bounded wait; the error flag records a give-up).
+__device__ __forceinline__ double await_value(const double *p, unsigned long long bits, int *err) {
+    unsigned polls = 0;
+    while (bits == PERS_EMPTY) {
+        __builtin_amdgcn_s_sleep(1);
+        bits = coh_bits(p);
+        if (++polls > PERS_POLLS) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            bits = 0;
+        }
+    }
+    return __longlong_as_double((long long)bits);
+}
 
-constexpr unsigned GRID_POLLS = 1u << 22;  // ~0.1 s of polling per barrier before giving up
-
-__device__ __forceinline__ void grid_barrier(unsigned long long *arrive, unsigned long long target, int *err) {
-    lds_barrier();  // every wave of the workgroup has completed its stores (each waited for them)
-    if (threadIdx.x == 0) {
-        __hip_atomic_fetch_add(arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned polls = 0;
-        while (__hip_atomic_load(arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(2);
-            if (++polls > GRID_POLLS) {
-                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                break;
+// Stages n doubles (zero past the given lengths) into LDS: all of a thread's loads are issued before
+// any is examined, so a ready vector costs one memory round trip per 2048 entries.  AWAIT: the entries
+// are written by earlier stages of this launch (polled until written).
+template <bool AWAIT>
+__device__ __forceinline__ void stage_vec(double *dst, int n, const double *a, int na, const double *b, int nb,
+                                          int *err) {
+    for (int j0 = 0; j0 < n; j0 += 8 * 256) {
+        unsigned long long v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int j = j0 + 256 * k + (int)threadIdx.x;
+            unsigned long long x = 0;
+            if (j < na) x = AWAIT ? coh_bits(a + j) : (unsigned long long)__double_as_longlong(gmem(a)[j]);
+            else if (j < na + nb) x = AWAIT ? coh_bits(b + (j - na)) : (unsigned long long)__double_as_longlong(gmem(b)[j - na]);
+            v[k] = x;
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int j = j0 + 256 * k + (int)threadIdx.x;
+            if (j < n) {
+                double x = __longlong_as_double((long long)v[k]);
+                if (AWAIT && v[k] == PERS_EMPTY) x = await_value(j < na ? a + j : b + (j - na), v[k], err);
+                dst[j] = x;
             }
         }
     }
-    lds_barrier();
+}
+
+__global__ void k_fill_empty(double *a, double *b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        a[i] = __longlong_as_double((long long)PERS_EMPTY);
+        b[i] = __longlong_as_double((long long)PERS_EMPTY);
+    }
 }
 
 template <int NK>
 __global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restrict__ stages, int nstages, int L1,
-                                                           int L2, unsigned long long *arrive,
-                                                           unsigned long long base, int *err) {
+                                                           int L2, int *err) {
     extern __shared__ double sv[];  // v1 staged in [0, L1), [v2a; v2b] in [L1, L1 + L2), zero past the ends
     constexpr int CH = 64 * NK;
     double *sv1 = sv, *sv2 = sv + L1;
-    const int G = gridDim.x, g = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // this wavefront's problem, first row and row stride in a stage
-    auto mine = [&](const PStage &S, int &qi, int &r0, int &rs) {
-        if (S.nq == 2 && g >= S.gA) {
-            qi = 1;
-            r0 = 4 * (g - S.gA) + w;
-            rs = 4 * (G - S.gA);
-        } else {
-            qi = 0;
-            r0 = 4 * g + w;
-            rs = 4 * (S.nq == 2 ? S.gA : G);
-        }
-    };
+    const int g = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    auto chunk_len = [&](int64_t ld) { return (int)(((ld + CH - 1) / CH) * CH); };
     auto load_chunk = [&](const double *row, int64_t ld, int64_t c0, double (&a)[NK]) {
 #pragma unroll
         for (int k = 0; k < NK; ++k) a[k] = (c0 + 64 * k < ld) ? gmem(row)[c0 + 64 * k + lane] : 0.0;
     };
-    double pre1[NK], pre2[NK];
-    auto prefetch = [&](int s) {
-        const PStage &S = stages[s];
-        int qi, r0, rs;
-        mine(S, qi, r0, rs);
-        const PRow &q = S.q[qi];
-        if (r0 < q.n) {
-            if (q.M1) load_chunk(q.M1 + (int64_t)r0 * q.ld1, q.ld1, 0, pre1);
-            if (q.M2) load_chunk(q.M2 + (int64_t)r0 * q.ld2, q.ld2, 0, pre2);
-        }
+    // A stage in flight: this wavefront's problem and row (one row per wavefront), whether the
+    // workgroup has rows at all, the M1 v1 partial sums and the first M2 chunk.
+    struct Cur {
+        int qi, r;
+        bool any;
+        double t1;
+        double pre2[NK];
     };
-    prefetch(0);
-    for (int s = 0; s < nstages; ++s) {
+    // Part A of stage s, which depends on no earlier stage: v1, the M1 v1 partial sums, the first M2 chunk.
+    auto part_a = [&](int s, Cur &c) {
         const PStage &S = stages[s];
-        int qi, r0, rs;
-        mine(S, qi, r0, rs);
-        const PRow &q = S.q[qi];
-        for (int j = threadIdx.x; j < L1; j += 256) sv1[j] = (q.M1 && j < q.n1) ? gmem(q.v1)[j] : 0.0;
-        for (int j = threadIdx.x; j < L2; j += 256) {
-            double v = 0.0;
-            if (q.M2) {
-                if (j < q.n2a) v = coh_load(q.v2a + j);
-                else if (j < q.n2a + q.n2b) v = coh_load(q.v2b + (j - q.n2a));
-            }
-            sv2[j] = v;
-        }
-        lds_barrier();
-        bool first = true;
-        for (int r = r0; r < q.n; r += rs) {
-            double t1 = 0.0, t2 = 0.0;
+        c.qi = (S.nq == 2 && g >= S.gA) ? 1 : 0;
+        c.r = 4 * (c.qi ? g - S.gA : g) + w;
+        const PRow &q = S.q[c.qi];
+        c.any = c.r - w < q.n;
+        c.t1 = 0.0;
+        if (c.any && q.M1) stage_vec<false>(sv1, chunk_len(q.ld1), q.v1, q.n1, q.v1, 0, err);
+        lds_barrier();  // unconditional: it also orders the previous part's sv2 reads before the next writes
+        if (c.r < q.n) {
             if (q.M1) {
-                const double *row = q.M1 + (int64_t)r * q.ld1;
+                const double *row = q.M1 + (int64_t)c.r * q.ld1;
                 for (int64_t c0 = 0; c0 < q.ld1; c0 += CH) {
                     double a[NK];
-                    if (first && c0 == 0) {
+                    load_chunk(row, q.ld1, c0, a);
 #pragma unroll
-                        for (int k = 0; k < NK; ++k) a[k] = pre1[k];
-                    } else {
-                        load_chunk(row, q.ld1, c0, a);
-                    }
-#pragma unroll
-                    for (int k = 0; k < NK; ++k) t1 = fma(a[k], sv1[c0 + 64 * k + lane], t1);
+                    for (int k = 0; k < NK; ++k) c.t1 = fma(a[k], sv1[c0 + 64 * k + lane], c.t1);
                 }
             }
+            if (q.M2) load_chunk(q.M2 + (int64_t)c.r * q.ld2, q.ld2, 0, c.pre2);
+        }
+    };
+    // Part B: the earlier stages' vectors, the M2 product and the store.
+    auto part_b = [&](int s, Cur &c) {
+        const PRow &q = stages[s].q[c.qi];
+        const int r = c.r;
+        double cval = 0.0;
+        unsigned long long cbits = 0;
+        if (q.c && lane == 0 && r < q.n) cbits = coh_bits(q.c + r);  // in the same round trip as the staging
+        if (c.any && q.M2) stage_vec<true>(sv2, chunk_len(q.ld2), q.v2a, q.n2a, q.v2b, q.n2b, err);
+        if (q.c && lane == 0 && r < q.n) cval = await_value(q.c + r, cbits, err);
+        lds_barrier();  // unconditional, like part A's
+        if (r < q.n) {
+            double t2 = 0.0;
             if (q.M2) {
                 const double *row = q.M2 + (int64_t)r * q.ld2;
                 for (int64_t c0 = 0; c0 < q.ld2; c0 += CH) {
                     double a[NK];
-                    if (first && c0 == 0) {
+                    if (c0 == 0) {
 #pragma unroll
-                        for (int k = 0; k < NK; ++k) a[k] = pre2[k];
+                        for (int k = 0; k < NK; ++k) a[k] = c.pre2[k];
                     } else {
                         load_chunk(row, q.ld2, c0, a);
                     }
@@ -599,16 +624,22 @@ __global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restr
                     for (int k = 0; k < NK; ++k) t2 = fma(a[k], sv2[c0 + 64 * k + lane], t2);
                 }
             }
-            double t = t1 - t2;
+            double t = c.t1 - t2;
             for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-            if (lane == 0) coh_store(q.y + r, (q.c ? coh_load(q.c + r) : 0.0) + t);
-            first = false;
+            if (lane == 0) coh_store(q.y + r, cval + t);
         }
-        if (s + 1 < nstages) {
-            __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
-            prefetch(s + 1);
-            grid_barrier(arrive, base + (unsigned long long)(s + 1) * (unsigned long long)G, err);
-        }
+    };
+    // Two stages in flight: part A of stage s + 1 (its loads) is issued before part B of stage s waits
+    // for its input.  LDS: part A writes only sv1 and part B only sv2, each between the other part's
+    // barrier and its own, so every write follows every wave's reads of the same region one part earlier
+    // (the barriers are taken also by workgroups without rows in a stage).
+    Cur cur, nxt;
+    part_a(0, cur);
+    lds_barrier();  // stage 0's sv1 reads before stage 1's sv1 writes (no part B lies between them)
+    for (int s = 0; s < nstages; ++s) {
+        if (s + 1 < nstages) part_a(s + 1, nxt);
+        part_b(s, cur);
+        cur = nxt;
     }
 }
 
@@ -722,6 +753,7 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
     Dk.alloc(2 * CB * CB);
     bb.alloc(std::max<int64_t>(ntot, 1));
     yy.alloc(std::max<int64_t>(ntot, 1));
+    xx.alloc(std::max<int64_t>(ntot, 1));
     info.alloc(P);
     pers_built = false;  // the persistent plan points into the buffers above
     TG_HIP(hipStreamSynchronize(st), "tg_solver:coarse");
@@ -891,10 +923,14 @@ void CoarseSolver::build_persistent(hipStream_t st) {
     const int nk = (int)((maxpad + 63) / 64);
     if (nk < 1 || nk > 16) return;  // planes wider than 1024 DOFs: step by step
     PRow none{};
-    auto row = [&](int p, bool diag, const double *M2, int64_t ld2, int qa, int qb, bool cz) {
+    // forward results (and the chains' inputs) in zz = yy, the separator's and backward results in xx
+    const double *zz = yy.p;
+    double *xxp = xx.p;
+    auto row = [&](int p, double *y, bool diag, const double *M2, int64_t ld2, const double *va, int qa,
+                   const double *vb, int qb, const double *c) {
         PRow q{};
         q.n = (int)nd[p];
-        q.y = yy.p + off[p];
+        q.y = y + off[p];
         if (diag) {
             q.M1 = Sinv.p + sOff[p];
             q.ld1 = npad[p];
@@ -905,19 +941,19 @@ void CoarseSolver::build_persistent(hipStream_t st) {
         if (M2 && (qa >= 0 || qb >= 0)) {
             q.M2 = M2;
             q.ld2 = ld2;
-            q.v2a = qa >= 0 ? yy.p + off[qa] : yy.p;
+            q.v2a = qa >= 0 ? va + off[qa] : va;
             q.n2a = qa >= 0 ? (int)nd[qa] : 0;
-            q.v2b = qb >= 0 ? yy.p + off[qb] : yy.p;
+            q.v2b = qb >= 0 ? vb + off[qb] : vb;
             q.n2b = qb >= 0 ? (int)nd[qb] : 0;
             maxld2 = std::max(maxld2, q.ld2);
         }
-        if (cz) q.c = yy.p + off[p];
+        if (c) q.c = c + off[p];
         return q;
     };
     std::vector<PStage> tab;
     auto add = [&](const PRow &a, const PRow &b) {
         PStage S{};
-        const bool ha = a.n > 0 && (a.M1 || a.M2), hb = b.n > 0 && (b.M1 || b.M2);
+        const bool ha = a.n > 0, hb = b.n > 0;
         if (!ha && !hb) return;
         S.q[0] = ha ? a : b;
         S.q[1] = ha && hb ? b : none;
@@ -928,17 +964,21 @@ void CoarseSolver::build_persistent(hipStream_t st) {
     for (int k = 0; k < std::max(lenA, lenB); ++k) {  // forward: z_p = S_p^-1 b_p - F_p z_{p-1|p+1}
         const int p = k, q = P - 1 - k;
         PRow a = none, b = none;
-        if (k < lenA && ok(p)) a = row(p, true, Fop.p + fOff[p], fLd[p], ok(p - 1) && k > 0 ? p - 1 : -1, -1, false);
-        if (k < lenB && ok(q)) b = row(q, true, Fop.p + fOff[q], fLd[q], ok(q + 1) && k > 0 ? q + 1 : -1, -1, false);
+        if (k < lenA && ok(p))
+            a = row(p, yy.p, true, Fop.p + fOff[p], fLd[p], zz, ok(p - 1) && k > 0 ? p - 1 : -1, zz, -1, nullptr);
+        if (k < lenB && ok(q))
+            b = row(q, yy.p, true, Fop.p + fOff[q], fLd[q], zz, ok(q + 1) && k > 0 ? q + 1 : -1, zz, -1, nullptr);
         add(a, b);
     }
     if (ok(m))  // separator: x_m = S_m^-1 b_m - F_m [z_{m-1}; z_{m+1}]
-        add(row(m, true, Fop.p + fOff[m], fLd[m], ok(m - 1) ? m - 1 : -1, ok(m + 1) ? m + 1 : -1, false), none);
+        add(row(m, xxp, true, Fop.p + fOff[m], fLd[m], zz, ok(m - 1) ? m - 1 : -1, zz, ok(m + 1) ? m + 1 : -1,
+                nullptr),
+            none);
     for (int k = 1; k <= std::max(lenA, lenB); ++k) {  // backward: x_p = z_p - B_p x_{p+1|p-1}
         const int p = m - k, q = m + k;
         PRow a = none, b = none;
-        if (k <= lenA && ok(p) && ok(p + 1)) a = row(p, false, Bop.p + bOff[p], bLd[p], p + 1, -1, true);
-        if (k <= lenB && ok(q) && ok(q - 1)) b = row(q, false, Bop.p + bOff[q], bLd[q], q - 1, -1, true);
+        if (k <= lenA && ok(p)) a = row(p, xxp, false, Bop.p + bOff[p], bLd[p], xxp, ok(p + 1) ? p + 1 : -1, xxp, -1, zz);
+        if (k <= lenB && ok(q)) b = row(q, xxp, false, Bop.p + bOff[q], bLd[q], xxp, ok(q - 1) ? q - 1 : -1, xxp, -1, zz);
         add(a, b);
     }
     if (tab.empty()) return;
@@ -962,20 +1002,21 @@ void CoarseSolver::build_persistent(hipStream_t st) {
     }
     if (per_cu < 1 || cus < 1) return;
     const int G = std::min(per_cu, 2) * cus;
-    for (PStage &S : tab)
+    for (PStage &S : tab) {  // one row per wavefront, or the step-by-step sequence
         if (S.nq == 2) {
             const int64_t na = S.q[0].n, nb = S.q[1].n;
             S.gA = (int)std::min<int64_t>(G - 1, std::max<int64_t>(1, (G * na + (na + nb) / 2) / (na + nb)));
+            if (na > 4 * (int64_t)S.gA || nb > 4 * (int64_t)(G - S.gA)) return;
+        } else if (S.q[0].n > 4 * (int64_t)G) {
+            return;
         }
+    }
     std::vector<unsigned char> bytes(tab.size() * sizeof(PStage));
     std::memcpy(bytes.data(), tab.data(), bytes.size());
     pers_table.upload(bytes, st);
-    pers_arrive.alloc(1);
-    pers_arrive.zero(st);
     pers_err.alloc(1);
     pers_err.zero(st);
     TG_HIP(hipStreamSynchronize(st), id);
-    pers_base = 0;
     pers_pending = false;
     pers_nk = nk;
     pers_groups = G;
@@ -989,11 +1030,11 @@ void CoarseSolver::enqueue_persistent(hipStream_t st) {
     const char *id = "tg_solver:coarse";
     const size_t lds = (size_t)(pers_L1 + pers_L2) * sizeof(double);
     const PStage *tab = (const PStage *)pers_table.p;
+    k_fill_empty<<<grid_for(ntot, 256), 256, 0, st>>>(yy, xx, ntot);
     switch (pers_nk) {
 #define TG_PERS_CASE(K)                                                                                   \
     case K:                                                                                               \
-        k_coarse_persistent<K><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_arrive.p, \
-                                                              pers_base, pers_err.p);                     \
+        k_coarse_persistent<K><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p); \
         break;
         TG_PERS_CASE(1) TG_PERS_CASE(2) TG_PERS_CASE(3) TG_PERS_CASE(4) TG_PERS_CASE(5) TG_PERS_CASE(6)
         TG_PERS_CASE(7) TG_PERS_CASE(8) TG_PERS_CASE(9) TG_PERS_CASE(10) TG_PERS_CASE(11) TG_PERS_CASE(12)
@@ -1002,7 +1043,6 @@ void CoarseSolver::enqueue_persistent(hipStream_t st) {
         default: fail(TG_ERR_ARGS, id, "bad persistent chunking");
     }
     TG_LAUNCH_CHECK(id);
-    pers_base += (unsigned long long)(pers_stages - 1) * (unsigned long long)pers_groups;
     pers_pending = true;
 }
 
@@ -1014,11 +1054,9 @@ void CoarseSolver::check_barrier(hipStream_t st) {
     TG_HIP(hipStreamSynchronize(st), id);
     pers_pending = false;
     if (e) {
-        pers_arrive.zero(st);
         pers_err.zero(st);
         TG_HIP(hipStreamSynchronize(st), id);
-        pers_base = 0;
-        fail(TG_ERR_HIP, id, "coarsest-level solve: a grid barrier of the persistent launch timed out");
+        fail(TG_ERR_HIP, id, "coarsest-level solve: a stage of the persistent launch timed out waiting for its input");
     }
 }
 
@@ -1029,7 +1067,8 @@ void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t 
     k_pack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(r, bb, nodeMask, lidx, d_off, g, axis);
     TG_LAUNCH_CHECK(id);
     if (use_persistent && !use_graph && !pers_built) build_persistent(st);
-    if (use_persistent && !use_graph && pers_ok) {
+    const bool pers = use_persistent && !use_graph && pers_ok;
+    if (pers) {
         enqueue_persistent(st);
     } else if (!use_graph) {
         enqueue_core(st);
@@ -1048,7 +1087,7 @@ void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t 
         }
         TG_HIP(hipGraphLaunch(exec, st), id);
     }
-    k_unpack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(yy, x, nodeMask, lidx, d_off, g, axis);
+    k_unpack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(pers ? xx : yy, x, nodeMask, lidx, d_off, g, axis);
     TG_LAUNCH_CHECK(id);
 }
 

@@ -311,9 +311,9 @@ def main():
     ap.add_argument('--kfine', choices=('walsh', 'mfma32'), default='walsh',
                     help='mixed solver: finest element products in the Walsh form on the VALU (default) or dense on '
                          'MFMA with FP32 operands (for comparison)')
-    ap.add_argument('--coarse', choices=('step', 'persistent'), default=None,
-                    help='coarsest solve: launched step by step or as one persistent launch (default: the '
-                         "library's default)")
+    ap.add_argument('--coarse', choices=('step', 'persistent', 'streaming'), default=None,
+                    help='coarsest solve: launched step by step, as one persistent launch, or persistent with '
+                         "non-temporal factor loads (default: the library's default)")
     ap.add_argument('--mex', type=int, default=0, metavar='REPS',
                     help='only time the six MEX entry points (REPS calls each) at the --size shapes')
     args = ap.parse_args()
@@ -337,7 +337,7 @@ def main():
     run.solvers['mixed'].set_tuning(kfine_mfma=args.kfine)
     if args.coarse:
         for sv in run.solvers.values():
-            sv.set_tuning(coarse_persistent=args.coarse == 'persistent')
+            sv.set_tuning(coarse_persistent=args.coarse != 'step', coarse_streaming=args.coarse == 'streaming')
     torch.cuda.synchronize()
     log('setup + solid solves %.1fs (Ne=%d)' % (time.perf_counter() - t, run.Ne))
     hist = []

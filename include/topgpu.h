@@ -237,11 +237,15 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *   TG_TUNE_COARSE_PERSISTENT  1 = run the coarsest solve as one persistent launch whose workgroups pass
  *                         the recurrence vectors to each other directly (default; planes of at most 1024
  *                         DOFs, no graph requested), 0 = step by step
+ *   TG_TUNE_COARSE_STREAMING  1 = the persistent coarsest solve reads its factors with non-temporal
+ *                         (streaming) loads (default; 174 against 178-185 us at 128 x 256 x 128),
+ *                         0 = ordinary loads
  *   TG_TUNE_KFINE_MFMA    element products of the mixed V-cycle's finest K*u: 0 = Walsh form on the
  *                         VALU (default, the fastest measured), 1 = dense 24 x 24 on MFMA f32 32x32x2,
  *                         2 = dense on MFMA f32 32x32x16 bf16 (BF16 operands, FP32 accumulation) */
 enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4,
-       TG_TUNE_COARSE_GRAPH = 5, TG_TUNE_KFINE_MFMA = 6, TG_TUNE_COARSE_PERSISTENT = 7 };
+       TG_TUNE_COARSE_GRAPH = 5, TG_TUNE_KFINE_MFMA = 6, TG_TUNE_COARSE_PERSISTENT = 7,
+       TG_TUNE_COARSE_STREAMING = 8 };
 int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value);
 /* Damped-Jacobi weight of the V-cycle smoother (the reference's global weightFactorJacobi_,
  * TOP_GPU.m:397, 784, 809-811; default 0.6).  0 < omega < 2.  The smoother only reduces every error

@@ -109,7 +109,7 @@ SIGNATURES = {
 }
 TG_EW_MUL, TG_EW_DIV_FLOOR, TG_EW_SET_WHERE = 0, 1, 2
 (TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK, TG_TUNE_TIME_KFINE, TG_TUNE_KELEM_PARTS, TG_TUNE_COARSE_GRAPH,
- TG_TUNE_KFINE_MFMA, TG_TUNE_COARSE_PERSISTENT) = 1, 2, 3, 4, 5, 6, 7
+ TG_TUNE_KFINE_MFMA, TG_TUNE_COARSE_PERSISTENT, TG_TUNE_COARSE_STREAMING) = 1, 2, 3, 4, 5, 6, 7, 8
 TG_RED_SUM, TG_RED_DOT, TG_RED_MAXABSDIFF, TG_RED_SHARPNESS = 0, 1, 2, 3
 
 _lib = None

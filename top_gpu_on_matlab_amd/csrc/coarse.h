@@ -55,6 +55,7 @@ struct CoarseSolver {
     // sequence above is used when this is off, when a graph is requested, or when the planes are too
     // wide for it (more than 1024 DOFs per plane)
     bool use_persistent = true;
+    bool pers_streaming = true;  // non-temporal loads of the factor rows
     bool pers_built = false, pers_ok = false;
     int pers_nk = 0, pers_groups = 0, pers_stages = 0, pers_L1 = 0, pers_L2 = 0;
     DevBuf<unsigned char> pers_table;        // the stage table (device copy)

@@ -555,7 +555,7 @@ __global__ void k_fill_empty(double *a, double *b, int64_t n) {
     }
 }
 
-template <int NK>
+template <int NK, bool NT>
 __global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restrict__ stages, int nstages, int L1,
                                                            int L2, int *err) {
     extern __shared__ double sv[];  // v1 staged in [0, L1), [v2a; v2b] in [L1, L1 + L2), zero past the ends
@@ -563,9 +563,14 @@ __global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restr
     double *sv1 = sv, *sv2 = sv + L1;
     const int g = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     auto chunk_len = [&](int64_t ld) { return (int)(((ld + CH - 1) / CH) * CH); };
+    // NT: the factor rows are read once per solve; streaming (non-temporal) loads keep them from
+    // displacing the finest level's data in the caches
     auto load_chunk = [&](const double *row, int64_t ld, int64_t c0, double (&a)[NK]) {
 #pragma unroll
-        for (int k = 0; k < NK; ++k) a[k] = (c0 + 64 * k < ld) ? gmem(row)[c0 + 64 * k + lane] : 0.0;
+        for (int k = 0; k < NK; ++k)
+            a[k] = (c0 + 64 * k < ld) ? (NT ? __builtin_nontemporal_load(gmem(row) + c0 + 64 * k + lane)
+                                            : gmem(row)[c0 + 64 * k + lane])
+                                      : 0.0;
     };
     // A stage in flight: this wavefront's problem and row (one row per wavefront), whether the
     // workgroup has rows at all, the M1 v1 partial sums and the first M2 chunk.
@@ -993,7 +998,7 @@ void CoarseSolver::build_persistent(hipStream_t st) {
     switch (nk) {
 #define TG_PERS_CASE(K)                                                                                   \
     case K:                                                                                               \
-        TG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coarse_persistent<K>, 256, lds), id); \
+        TG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_coarse_persistent<K, true>, 256, lds), id); \
         break;
         TG_PERS_CASE(1) TG_PERS_CASE(2) TG_PERS_CASE(3) TG_PERS_CASE(4) TG_PERS_CASE(5) TG_PERS_CASE(6)
         TG_PERS_CASE(7) TG_PERS_CASE(8) TG_PERS_CASE(9) TG_PERS_CASE(10) TG_PERS_CASE(11) TG_PERS_CASE(12)
@@ -1034,7 +1039,10 @@ void CoarseSolver::enqueue_persistent(hipStream_t st) {
     switch (pers_nk) {
 #define TG_PERS_CASE(K)                                                                                   \
     case K:                                                                                               \
-        k_coarse_persistent<K><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p); \
+        if (pers_streaming)                                                                                \
+            k_coarse_persistent<K, true><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p); \
+        else                                                                                              \
+            k_coarse_persistent<K, false><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p); \
         break;
         TG_PERS_CASE(1) TG_PERS_CASE(2) TG_PERS_CASE(3) TG_PERS_CASE(4) TG_PERS_CASE(5) TG_PERS_CASE(6)
         TG_PERS_CASE(7) TG_PERS_CASE(8) TG_PERS_CASE(9) TG_PERS_CASE(10) TG_PERS_CASE(11) TG_PERS_CASE(12)

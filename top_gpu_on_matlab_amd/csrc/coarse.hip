@@ -321,27 +321,36 @@ struct DSProb {
     int64_t ld, col0;
 };
 
+// One thread per output column 3 jn + c and run of DS_ROWS rows: the column's 27 couplings and their
+// columns stay in registers over the run, the rows' gathered entries come from the cache (the lanes of
+// a wavefront share the row), and lanes write consecutive columns.  (A thread per row and column node
+// re-read the couplings for every row: 167 us per launch at 128 x 256 x 128.)
+constexpr int DS_ROWS = 16;
 __global__ __launch_bounds__(256) void k_dense_sparse(const DSProb P0, const DSProb P1, const DSProb P2,
                                                       const DSProb P3) {
     const DSProb &q = blockIdx.y == 0 ? P0 : (blockIdx.y == 1 ? P1 : (blockIdx.y == 2 ? P2 : P3));
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= q.nrows * q.ncn) return;
-    const int64_t i = t / q.ncn, jn = t - i * q.ncn;
-    const double *srow = q.S + i * q.np;
-    const int32_t *C = q.C + 3 * jn * NCPL;
-    const double *V = q.V + 3 * jn * NCPL;
-    double o0 = 0.0, o1 = 0.0, o2 = 0.0;
-#pragma unroll 9
+    const int64_t ncol = 3 * q.ncn, nruns = (q.nrows + DS_ROWS - 1) / DS_ROWS;
+    if (t >= ncol * nruns) return;
+    const int64_t jc = t % ncol, i0 = (t / ncol) * DS_ROWS;
+    const int64_t jn = jc / 3;
+    const int32_t *C = q.C + 3 * jn * NCPL;  // the three DOFs of node jn share their column pattern
+    const double *V = q.V + jc * NCPL;
+    int32_t ck[NCPL];
+    double vk[NCPL];
+#pragma unroll
     for (int k = 0; k < NCPL; ++k) {
-        const double sv = srow[C[k]];
-        o0 += sv * V[k];
-        o1 += sv * V[NCPL + k];
-        o2 += sv * V[2 * NCPL + k];
+        ck[k] = C[k];
+        vk[k] = V[k];
     }
-    double *o = q.Out + i * q.ld + q.col0 + 3 * jn;
-    o[0] = o0;
-    o[1] = o1;
-    o[2] = o2;
+    const int64_t i1 = i0 + DS_ROWS < q.nrows ? i0 + DS_ROWS : q.nrows;
+    for (int64_t i = i0; i < i1; ++i) {
+        const double *srow = q.S + i * q.np;
+        double o = 0.0;
+#pragma unroll
+        for (int k = 0; k < NCPL; ++k) o += srow[ck[k]] * vk[k];
+        q.Out[i * q.ld + q.col0 + jc] = o;
+    }
 }
 
 // c = blockdiag(S_p^-1) b over all planes: one wavefront per DOF row.
@@ -464,8 +473,9 @@ void launch_chain(const ChainProb &A, const ChainProb &B, hipStream_t s) {
 // workgroups in proportion to their rows).  The forward stages fold the block-diagonal product into
 // the recurrence, z_p = S_p^-1 b_p - F_p z_{p-1}, and write zz; the separator and the backward stages,
 // x_p = z_p - B_p x_{p+1}, write xx.
-// Dataflow instead of barriers: before the launch zz and xx are filled with a sentinel bit pattern (a
-// signalling NaN that arithmetic never produces), every entry is written once per launch, and a
+// Dataflow instead of barriers: before the launch (by the kernel that packs the right-hand side) zz and
+// xx are filled with a sentinel bit pattern (a signalling NaN that arithmetic never produces), every
+// entry is written once per launch, and a
 // workgroup that needs an earlier stage's vector polls exactly the entries it stages until they are no
 // longer the sentinel.  Stores and these loads are device-scope atomic accesses, coherent across the
 // XCDs' separate L2 caches.  (A grid barrier between stages measured 2.6 us with per-group counters and
@@ -544,14 +554,6 @@ __device__ __forceinline__ void stage_vec(double *dst, int n, const double *a, i
                 dst[j] = x;
             }
         }
-    }
-}
-
-__global__ void k_fill_empty(double *a, double *b, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        a[i] = __longlong_as_double((long long)PERS_EMPTY);
-        b[i] = __longlong_as_double((long long)PERS_EMPTY);
     }
 }
 
@@ -650,7 +652,8 @@ __global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restr
 
 template <typename T>
 __global__ void k_pack_planes(const T *__restrict__ f, double *__restrict__ b, const uint8_t *__restrict__ mask,
-                              const int32_t *__restrict__ lidx, const int64_t *__restrict__ off, const Grid g, int axis) {
+                              const int32_t *__restrict__ lidx, const int64_t *__restrict__ off, const Grid g, int axis,
+                              double *__restrict__ e1 = nullptr, double *__restrict__ e2 = nullptr) {
     const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= g.nodes() || !(mask[n] & NODE_ACTIVE)) return;
     const int y = (int)(n % g.NY);
@@ -660,6 +663,13 @@ __global__ void k_pack_planes(const T *__restrict__ f, double *__restrict__ b, c
     b[base] = (double)f[3 * n];
     b[base + 1] = (double)f[3 * n + 1];
     b[base + 2] = (double)f[3 * n + 2];
+    if (e1) {  // the persistent solve's vectors start out "not written" (every plane slot is an active node's)
+        const double empty = __longlong_as_double((long long)PERS_EMPTY);
+        for (int c = 0; c < 3; ++c) {
+            e1[base + c] = empty;
+            e2[base + c] = empty;
+        }
+    }
 }
 
 template <typename T>
@@ -840,7 +850,7 @@ void CoarseSolver::factor(const double *Ks, int64_t ldk, const uint8_t *eleMask,
             const int cnt = (int)std::min<size_t>(4, ops.size() - i0);
             for (int y = 0; y < cnt; ++y) {
                 q[y] = ops[i0 + y];
-                work = std::max(work, q[y].nrows * q[y].ncn);
+                work = std::max(work, 3 * q[y].ncn * ((q[y].nrows + DS_ROWS - 1) / DS_ROWS));
             }
             if (work) k_dense_sparse<<<dim3(grid_for(work, 256), cnt), 256, 0, st>>>(q[0], q[1], q[2], q[3]);
         }
@@ -1035,7 +1045,6 @@ void CoarseSolver::enqueue_persistent(hipStream_t st) {
     const char *id = "tg_solver:coarse";
     const size_t lds = (size_t)(pers_L1 + pers_L2) * sizeof(double);
     const PStage *tab = (const PStage *)pers_table.p;
-    k_fill_empty<<<grid_for(ntot, 256), 256, 0, st>>>(yy, xx, ntot);
     switch (pers_nk) {
 #define TG_PERS_CASE(K)                                                                                   \
     case K:                                                                                               \
@@ -1072,10 +1081,11 @@ template <typename T>
 void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st) {
     const char *id = "tg_solver:coarse";
     if (!factored) fail(TG_ERR_STATE, id, "coarsest level not factorised (call assemble first)");
-    k_pack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(r, bb, nodeMask, lidx, d_off, g, axis);
-    TG_LAUNCH_CHECK(id);
     if (use_persistent && !use_graph && !pers_built) build_persistent(st);
     const bool pers = use_persistent && !use_graph && pers_ok;
+    k_pack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(r, bb, nodeMask, lidx, d_off, g, axis,
+                                                               pers ? yy.p : nullptr, pers ? xx.p : nullptr);
+    TG_LAUNCH_CHECK(id);
     if (pers) {
         enqueue_persistent(st);
     } else if (!use_graph) {

@@ -50,6 +50,15 @@ def measured_traffic(shape, kernel='fp64_pcg_operator'):
     return None
 
 
+# the mixed solver with its finest element products on the matrix cores in BF16 (reported beside FP32)
+BF16_MODES = ('bf16', 'bf16x3')
+BF16_MODES_TEXT = {
+    'bf16': 'dense 24x24 on MFMA f32 32x32x16 bf16 (BF16 operands, FP32 accumulation), rest of the mixed V-cycle FP32',
+    'bf16x3': 'dense 24x24 on MFMA f32 32x32x16 bf16 with each operand split into two BF16 parts (hi lo + lo hi + '
+              'hi hi, FP32 accumulation), rest of the mixed V-cycle FP32',
+}
+
+
 def log(*a):
     print('[bench]', *a, file=sys.stderr, flush=True)
 
@@ -72,9 +81,11 @@ class Run:
         self.G, self.L, self.torch = G, _lib, torch
         self.V0, self.Emin = V0, Emin
         self.models = {p: cuboid_model(*shape, 1, precision=p) for p in ('fp64', 'mixed')}
-        self.models['bf16'] = cuboid_model(*shape, 1, precision='mixed')
+        for p in BF16_MODES:  # the mixed solver with its finest element products as BF16 on MFMA
+            self.models[p] = cuboid_model(*shape, 1, precision='mixed')
         self.solvers = {p: Solver(self.models[p]) for p in self.models}
-        self.solvers['bf16'].set_tuning(kfine_mfma='bf16')
+        for p in BF16_MODES:
+            self.solvers[p].set_tuning(kfine_mfma=p)
         self.model = self.models['fp64']
         self.filt = G.PDEFilter(self.model, rMin, tol=1e-4, max_it=800)
         self.Ne = self.model.num_elements
@@ -108,7 +119,7 @@ class Run:
         G, L, torch = self.G, self.L, self.torch
         t0 = time.perf_counter()
         E = G.simp(self.xPhys, self.Emin, 1.0, 3.0)
-        r = {p: self.solve(p, E) for p in ('fp64', 'mixed', 'bf16')}
+        r = {p: self.solve(p, E) for p in ('fp64', 'mixed') + BF16_MODES}
         t1 = time.perf_counter()
         ce = G.unit_compliance(self.solvers['fp64'], self.U['fp64'])
         dc = G.sensitivity(self.xPhys, ce, self.cSolid, self.Emin, 1.0, 3.0)
@@ -121,7 +132,7 @@ class Run:
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         r['update_s'] = t2 - t1
-        r['iter_s'] = t2 - t0 - sum(r[p]['pcg_s'] + r[p]['assembly_s'] for p in ('mixed', 'bf16'))
+        r['iter_s'] = t2 - t0 - sum(r[p]['pcg_s'] + r[p]['assembly_s'] for p in ('mixed',) + BF16_MODES)
         return r
 
 
@@ -349,22 +360,24 @@ def main():
             run.solvers['mixed'].time_kfine(True, only_operator=True)
         r = run.iterate()
         hist.append(r)
-        log('it %d: fp64 PCG %.1f ms (%d its) c=%.12e | mixed PCG %.1f ms (%d its) c=%.12e | rel %.1e | '
-            'bf16 PCG %.1f ms (%d its) rel %.1e | assembly %.0f ms, FP64 iteration %.0f ms'
+        log('it %d: fp64 PCG %.1f ms (%d its) c=%.12e | mixed PCG %.1f ms (%d its) c=%.12e | rel %.1e | %s | '
+            'assembly %.0f ms, FP64 iteration %.0f ms'
             % (k + 1, 1e3 * r['fp64']['pcg_s'], r['fp64']['its'], r['fp64']['c'], 1e3 * r['mixed']['pcg_s'],
                r['mixed']['its'], r['mixed']['c'], abs(r['mixed']['c'] - r['fp64']['c']) / abs(r['fp64']['c']),
-               1e3 * r['bf16']['pcg_s'], r['bf16']['its'], abs(r['bf16']['c'] - r['fp64']['c']) / abs(r['fp64']['c']),
+               ' | '.join('%s PCG %.1f ms (%d its) rel %.1e' % (p, 1e3 * r[p]['pcg_s'], r[p]['its'],
+                                                                abs(r[p]['c'] - r['fp64']['c']) / abs(r['fp64']['c']))
+                          for p in BF16_MODES),
                1e3 * r['fp64']['assembly_s'], 1e3 * r['iter_s']))
     timed = hist[args.warmup:]
     res = {}
-    for p in ('fp64', 'mixed', 'bf16'):
+    for p in ('fp64', 'mixed') + BF16_MODES:
         pcg = sum(h[p]['pcg_s'] for h in timed)
         res[p] = dict(solves_per_s=len(timed) / pcg, ms_per_solve=1e3 * pcg / len(timed),
                       pcg_its=[h[p]['its'] for h in timed], assembly_ms=1e3 * np.mean([h[p]['assembly_s'] for h in timed]),
                       compliance=[h[p]['c'] for h in timed])
     res['fp64']['iteration_ms'] = 1e3 * np.mean([h['iter_s'] for h in timed])
     errs = [abs(h['mixed']['c'] - h['fp64']['c']) / abs(h['fp64']['c']) for h in timed]
-    errs16 = [abs(h['bf16']['c'] - h['fp64']['c']) / abs(h['fp64']['c']) for h in timed]
+    errs16 = {p: [abs(h[p]['c'] - h['fp64']['c']) / abs(h['fp64']['c']) for h in timed] for p in BF16_MODES}
     # finest K*u: average launch duration over the timed steps (HIP events around every launch on the
     # solver stream); FP64 = the PCG operator of the FP64 solver, FP32 = the mixed V-cycle's residual
     # and smoothing applies
@@ -410,10 +423,10 @@ def main():
         'mixed': res['mixed'],
         'mixed_over_fp64': res['mixed']['solves_per_s'] / res['fp64']['solves_per_s'],
         'rel_compliance_err_mixed_vs_fp64': {'max': max(errs), 'per_iteration': errs},
-        'mixed_bf16': dict(res['bf16'], element_products='dense 24x24 on MFMA f32 32x32x16 bf16 (BF16 operands, '
-                                                         'FP32 accumulation), rest of the mixed V-cycle FP32',
-                           over_fp64=res['bf16']['solves_per_s'] / res['fp64']['solves_per_s'],
-                           rel_compliance_err_vs_fp64={'max': max(errs16), 'per_iteration': errs16}),
+        **{'mixed_' + p: dict(res[p], element_products=BF16_MODES_TEXT[p],
+                              over_fp64=res[p]['solves_per_s'] / res['fp64']['solves_per_s'],
+                              rel_compliance_err_vs_fp64={'max': max(errs16[p]), 'per_iteration': errs16[p]})
+           for p in BF16_MODES},
         'kbyu': kst,
         'roofline': {'kernel': 'finest-level K*u, FP64, PCG operator with the fused p.Kp (k_kfine_walsh<double,0,true,true,false,0>), '
                                'launches of the mixed solver in the timed steps', 'bound': 'hbm',

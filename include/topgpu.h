@@ -242,7 +242,8 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *                         0 = ordinary loads
  *   TG_TUNE_KFINE_MFMA    element products of the mixed V-cycle's finest K*u: 0 = Walsh form on the
  *                         VALU (default, the fastest measured), 1 = dense 24 x 24 on MFMA f32 32x32x2,
- *                         2 = dense on MFMA f32 32x32x16 bf16 (BF16 operands, FP32 accumulation) */
+ *                         2 = dense on MFMA f32 32x32x16 bf16 (BF16 operands, FP32 accumulation),
+ *                         3 = as 2 with each operand split into two BF16 parts (three products) */
 enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4,
        TG_TUNE_COARSE_GRAPH = 5, TG_TUNE_KFINE_MFMA = 6, TG_TUNE_COARSE_PERSISTENT = 7,
        TG_TUNE_COARSE_STREAMING = 8 };

@@ -126,9 +126,11 @@ def test_mixed_precision_cantilever_96x192x96_against_fp64():
     """The mixed-precision configuration at full length: 96 x 192 x 96 cantilever, ft = 2, rMin = 3,
     V0 = 0.12, 50 optimisation iterations driven by the FP64 solver; every iteration's system is solved
     by the mixed solver too (FP32 V-cycle, FP64 PCG), warm-started from its own previous solution, and by
-    the mixed solver with the finest element products as BF16 operands on the matrix cores.  Per
-    iteration, FP32 V-cycle: relative compliance difference within 1e-6 (measured ~1e-11) and the same
-    PCG count, or one more or fewer; the BF16 solver's differences are reported."""
+    the mixed solver with the finest element products as BF16 operands on the matrix cores, once with
+    each operand as one BF16 value and once split into two (hi + lo).  Per iteration, the FP32 V-cycle and
+    the split BF16 products: relative compliance difference within 1e-6 (measured ~1e-11 and up to
+    4.8e-7) and the same PCG count, or one more or fewer; the single-BF16 solver's differences (up to
+    ~1e-5, up to 13 more PCG iterations) are reported."""
     import torch
     from top_gpu_on_matlab_amd import _lib
     from top_gpu_on_matlab_amd import top_gpu as G
@@ -138,6 +140,8 @@ def test_mixed_precision_cantilever_96x192x96_against_fp64():
     sv = {p: Solver(cuboid_model(*shape, 1, precision=p)) for p in ('fp64', 'mixed')}
     sv['bf16'] = Solver(cuboid_model(*shape, 1, precision='mixed'))
     sv['bf16'].set_tuning(kfine_mfma='bf16')
+    sv['bf16x3'] = Solver(cuboid_model(*shape, 1, precision='mixed'))
+    sv['bf16x3'].set_tuning(kfine_mfma='bf16x3')
     model = sv['fp64'].model
     Ne = model.num_elements
     F = torch.from_numpy(model.get_table(1, 'F')[:, 0].copy()).cuda()
@@ -146,7 +150,7 @@ def test_mixed_precision_cantilever_96x192x96_against_fp64():
     xPhys = x.clone()
     U = {p: None for p in sv}
     ones = torch.ones_like(x)
-    errs, its_hist, errs16 = [], [], []
+    errs, its_hist, errs16, errs163 = [], [], [], []
     for it in range(50):
         E = G.simp(xPhys, 1e-6, 1.0, 3.0)
         c, its = {}, {}
@@ -157,7 +161,8 @@ def test_mixed_precision_cantilever_96x192x96_against_fp64():
             c[p] = float(torch.dot(F, U[p]))
         errs.append(abs(c['mixed'] - c['fp64']) / c['fp64'])
         errs16.append(abs(c['bf16'] - c['fp64']) / c['fp64'])
-        its_hist.append((its['fp64'], its['mixed'], its['bf16']))
+        errs163.append(abs(c['bf16x3'] - c['fp64']) / c['fp64'])
+        its_hist.append((its['fp64'], its['mixed'], its['bf16'], its['bf16x3']))
         ce = G.unit_compliance(sv['fp64'], U['fp64'])
         dc = filt(G.sensitivity(xPhys, ce, 1.0, 1e-6, 1.0, 3.0))
         dv = filt(ones)
@@ -165,10 +170,14 @@ def test_mixed_precision_cantilever_96x192x96_against_fp64():
         x, _ = G.oc_update(x, dc, dv, fval, 0.2)
         xPhys = filt(x)
     print('\n[96x192x96 mixed vs fp64, 50 iterations] rel compliance difference per iteration: FP32 V-cycle %s; '
-          'BF16 element products %s; PCG its (fp64, mixed, bf16) %s'
-          % (['%.1e' % e for e in errs], ['%.1e' % e for e in errs16], its_hist), flush=True)
+          'BF16 element products %s; split BF16 %s; PCG its (fp64, mixed, bf16, bf16x3) %s'
+          % (['%.1e' % e for e in errs], ['%.1e' % e for e in errs16], ['%.1e' % e for e in errs163], its_hist),
+          flush=True)
     assert max(errs) < 1e-6
-    assert all(abs(a - b) <= 1 for a, b, _ in its_hist), its_hist
+    assert all(abs(a - b) <= 1 for a, b, _, _ in its_hist), its_hist
+    # split BF16 (measured: within 4.8e-7, the FP64 counts or one more)
+    assert max(errs163) < 1e-6
+    assert all(abs(a - d) <= 1 for a, _, _, d in its_hist), its_hist
 
 
 @pytest.mark.parametrize('non_dyadic', [1, 0])

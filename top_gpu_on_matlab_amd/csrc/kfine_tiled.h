@@ -147,7 +147,9 @@ __device__ __forceinline__ void walsh_inv(T x[8]) {
 // MF (FP32 only; 0 = the Walsh form): the element product runs on the matrix cores as a dense 24 x 24
 // product (mfma_K0: the element matrix, cellSize included, column-major FP32; kw and cs are unused).
 // MF = 1: MFMA f32 32x32x2, FP32 operands.  MF = 2: MFMA f32 32x32x16 bf16, operands (E_e u_e and K0)
-// rounded to BF16, products accumulated in FP32.  Wave w takes the layer's elements 64 w .. 64 w + 63 as
+// rounded to BF16, products accumulated in FP32.  MF = 3: as 2 with each operand split into two BF16
+// parts (a = hi + lo, lo = bf16(a - hi), 16 significant bits together) and three products per block,
+// hi lo + lo hi + hi hi (lo lo, below 2^-16 of the product, is dropped).  Wave w takes the layer's elements 64 w .. 64 w + 63 as
 // two blocks of 32 (columns of the product).  Lane (n, h) (n = lane % 32, h = lane / 32) supplies the 12
 // values of face h (h = 0: node plane ez, 1: ez + 1) of element n, read from LDS: with MF = 1 as input
 // DOFs 2 s + h (value s = vertex s / 3 of the face, component s % 3), 12 steps per block; with MF = 2 as
@@ -162,7 +164,7 @@ __device__ __forceinline__ void walsh_inv(T x[8]) {
 // against 94 / 89 us for the residual / smoothing applies): 1536 padded flops per element on the FP32
 // matrix pipe take >= 58 us alone, where the Walsh form spends ~250 VALU operations per element.
 template <typename T, int EPI, bool CK, bool DOT, bool R64 = false, int MF = 0>
-__global__ __launch_bounds__(256, (sizeof(T) == 8 || MF != 0) ? 3 : 4) void k_kfine_walsh(const T *__restrict__ u, T *__restrict__ out,
+__global__ __launch_bounds__(256, MF == 3 ? 2 : ((sizeof(T) == 8 || MF != 0) ? 3 : 4)) void k_kfine_walsh(const T *__restrict__ u, T *__restrict__ out,
                                                      const T *__restrict__ E, const uint8_t *__restrict__ mask,
                                                      const KWVals<T> kw, T cs, const Grid g, const KTiles kt,
                                                      const T *__restrict__ r, const T *__restrict__ diag, T omega,
@@ -289,7 +291,8 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 || MF != 0) ? 3 : 4) void k_kf
     typedef __bf16 mf_bf8 __attribute__((ext_vector_type(8)));
     const int ml = t & 63, mn = ml & 31, mh = ml >> 5, mw = t >> 6;
     float mA[MF == 1 ? 2 : 1][MF == 1 ? 12 : 1];
-    mf_bf8 mAb[MF == 2 ? 2 : 1][MF == 2 ? 2 : 1];
+    mf_bf8 mAb[MF >= 2 ? 2 : 1][MF >= 2 ? 2 : 1];
+    mf_bf8 mAl[MF == 3 ? 2 : 1][MF == 3 ? 2 : 1];  // MF = 3: the low BF16 parts (a = hi + lo)
     float mkept[MF ? 2 : 1][MF ? 12 : 1];
     if constexpr (MF != 0) {
         const int g8 = mn >> 3, hh = (mn >> 2) & 1, ii = mn & 3, j = 4 * g8 + ii;  // output row mn
@@ -310,7 +313,11 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 || MF != 0) ? 3 : 4) void k_kf
 #pragma unroll
                 for (int s2 = 0; s2 < 2; ++s2)
 #pragma unroll
-                    for (int jj = 0; jj < 8; ++jj) mAb[P][s2][jj] = (__bf16)aval(P, 8 * s2 + jj);
+                    for (int jj = 0; jj < 8; ++jj) {
+                        const float a = aval(P, 8 * s2 + jj);
+                        mAb[P][s2][jj] = (__bf16)a;
+                        if constexpr (MF == 3) mAl[P][s2][jj] = (__bf16)(a - (float)mAb[P][s2][jj]);
+                    }
             }
         }
 #pragma unroll
@@ -377,6 +384,18 @@ __global__ __launch_bounds__(256, (sizeof(T) == 8 || MF != 0) ? 3 : 4) void k_kf
                     for (int jj = 0; jj < 8; ++jj) {
                         b0[jj] = (__bf16)bv[jj];
                         b1[jj] = (__bf16)(jj < 4 ? bv[8 + jj] : 0.0f);
+                    }
+                    if constexpr (MF == 3) {  // the cross terms first: hi lo + lo hi, then hi hi
+                        mf_bf8 l0, l1;
+#pragma unroll
+                        for (int jj = 0; jj < 8; ++jj) {
+                            l0[jj] = (__bf16)(bv[jj] - (float)b0[jj]);
+                            l1[jj] = (__bf16)(jj < 4 ? bv[8 + jj] - (float)b1[jj] : 0.0f);
+                        }
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P ? mAl[1][0] : mAl[0][0], b0, acc[b], 0, 0, 0);
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P ? mAl[1][1] : mAl[0][1], b1, acc[b], 0, 0, 0);
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P ? mAb[1][0] : mAb[0][0], l0, acc[b], 0, 0, 0);
+                        acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P ? mAb[1][1] : mAb[0][1], l1, acc[b], 0, 0, 0);
                     }
                     acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P ? mAb[1][0] : mAb[0][0], b0, acc[b], 0, 0, 0);
                     acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(P ? mAb[1][1] : mAb[0][1], b1, acc[b], 0, 0, 0);

@@ -562,6 +562,17 @@ static void launch_walsh_k(int epi, unsigned gt, hipStream_t st, const T *u, T *
                 k_kfine_walsh<T, EPI_JACOBI, false, DOT, false, 2><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
             return;
         }
+        if (mk && mf == 3) {  // as 2, each operand split into two BF16 parts (three products)
+            if (epi == EPI_KU)
+                k_kfine_walsh<T, EPI_KU, false, DOT, false, 3><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            else if (epi == EPI_RESID)
+                k_kfine_walsh<T, EPI_RESID, false, false, false, 3><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            else if (epi == EPI_RESID_JAC0)
+                k_kfine_walsh<T, EPI_RESID_JAC0, false, false, false, 3><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            else
+                k_kfine_walsh<T, EPI_JACOBI, false, DOT, false, 3><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr, mk);
+            return;
+        }
     }
     if (epi == EPI_KU)
         k_kfine_walsh<T, EPI_KU, CK, DOT><<<gt, 256, 0, st>>>(u, out, E, mask, kw, cs, g, kt, r, d, w, partial, xout, nullptr);
@@ -750,6 +761,10 @@ void Solver::vcycle_t(const T *r0, T *z0) {
                     (float)omega, dotp.p, nullptr, r64_dot, K0_32.p);
             else if (kfine_mfma == 2)
                 k_kfine_walsh<float, EPI_JACOBI, false, true, true, 2><<<gt, 256, 0, st>>>(
+                    x32, (float *)out, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32,
+                    (float)omega, dotp.p, nullptr, r64_dot, K0_32.p);
+            else if (kfine_mfma == 3)
+                k_kfine_walsh<float, EPI_JACOBI, false, true, true, 3><<<gt, 256, 0, st>>>(
                     x32, (float *)out, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32,
                     (float)omega, dotp.p, nullptr, r64_dot, K0_32.p);
             else if (const_k0)
@@ -1060,7 +1075,7 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
         else if (key == TG_TUNE_COARSE_GRAPH && (value == 0 || value == 1)) S.coarse.use_graph = value != 0;
         else if (key == TG_TUNE_COARSE_PERSISTENT && (value == 0 || value == 1)) S.coarse.use_persistent = value != 0;
         else if (key == TG_TUNE_COARSE_STREAMING && (value == 0 || value == 1)) S.coarse.pers_streaming = value != 0;
-        else if (key == TG_TUNE_KFINE_MFMA && value >= 0 && value <= 2) S.kfine_mfma = value;
+        else if (key == TG_TUNE_KFINE_MFMA && value >= 0 && value <= 3) S.kfine_mfma = value;
         else if (key == TG_TUNE_TIME_KFINE && value >= 0 && value <= 2) {
             S.time_fine = value;
             S.tused = 0;

@@ -128,8 +128,9 @@ class Solver:
         (True, default; coarse_streaming=True, the default, reads its factors with non-temporal loads) or step by step,
         then either replayed as a captured HIP graph (coarse_graph=True)
         or launched kernel by kernel; the mixed V-cycle's finest element products: 'walsh' (the Walsh
-        form on the VALU, default), 'mfma32' (dense, MFMA with FP32 operands) or 'bf16' (dense, MFMA with
-        BF16 operands and FP32 accumulation)."""
+        form on the VALU, default), 'mfma32' (dense, MFMA with FP32 operands), 'bf16' (dense, MFMA with
+        BF16 operands and FP32 accumulation) or 'bf16x3' (as 'bf16' with each operand split into two BF16
+        parts, three products)."""
         from ._lib import (TG_TUNE_COARSE_GRAPH, TG_TUNE_COARSE_PERSISTENT, TG_TUNE_COARSE_STREAMING,
                            TG_TUNE_KELEM_PARTS, TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_MFMA, TG_TUNE_KFINE_ZCHUNK)
         if coarse_streaming is not None:
@@ -137,7 +138,8 @@ class Solver:
         if coarse_persistent is not None:
             check(lib().tg_solver_set_tuning(self.h, TG_TUNE_COARSE_PERSISTENT, int(bool(coarse_persistent))))
         if kfine_mfma is not None:
-            mode = {'walsh': 0, 'mfma32': 1, 'bf16': 2}[kfine_mfma] if isinstance(kfine_mfma, str) else int(kfine_mfma)
+            mode = ({'walsh': 0, 'mfma32': 1, 'bf16': 2, 'bf16x3': 3}[kfine_mfma] if isinstance(kfine_mfma, str)
+                    else int(kfine_mfma))
             check(lib().tg_solver_set_tuning(self.h, TG_TUNE_KFINE_MFMA, mode))
         if coarse_graph is not None:
             check(lib().tg_solver_set_tuning(self.h, TG_TUNE_COARSE_GRAPH, int(bool(coarse_graph))))

@@ -225,6 +225,104 @@ def traffic_probe(shape):
     log('copies done (2 GiB read + 2 GiB write each), reductions (2 GiB read), fills (2 GiB write)')
 
 
+def device_copy_gbs(reps=5):
+    """Read + write GB/s of a 2 GiB device-to-device copy (torch copy_, HIP events): the practical HBM
+    ceiling that the published 8 TB/s peak is not."""
+    import torch
+    a = torch.empty(256 * 1024 * 1024, dtype=torch.float64, device='cuda')
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    gbs = 2 * a.numel() * 8 * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return gbs
+
+
+def kbyu_microbench(reps):
+    """The finest-level K*u at every configuration's shape, as the scope table's micro-benchmark asks:
+    moduli E = Emin + x^3 (1 - Emin) with x ~ U(0,1) (seed 2) on the solid elements, HIP events around
+    `reps` launches after one warm-up launch, FP64 and the mixed solver's FP32 operator (the Walsh form,
+    and the dense products on MFMA with BF16 and split-BF16 operands); bytes_alg per launch as in
+    kbyu_alg_bytes.  The input vector is the solver's level-1 work vector: the kernels' cost does not
+    depend on the values.  Beside them, the practical HBM ceilings: a 2 GiB device copy and a 2 GiB
+    read-only reduction."""
+    import tempfile
+    import torch
+    from top_gpu_on_matlab_amd.model import Model, cuboid_model
+    from top_gpu_on_matlab_amd.solver import Solver
+    from top_gpu_on_matlab_amd.synthetic import bottom_clamp_top_patch, value_noise_mask
+    from top_gpu_on_matlab_amd.topvoxel import load_topvoxel_model, write_topvoxel
+    out = {}
+    a = torch.empty(256 * 1024 * 1024, dtype=torch.float64, device='cuda')
+    a.uniform_(-1.0, 1.0)
+    b = torch.empty_like(a)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    b.copy_(a)
+    ev[0].record()
+    for _ in range(10):
+        b.copy_(a)
+    ev[1].record()
+    torch.cuda.synchronize()
+    out['copy_2GiB'] = {'GBs': 2 * a.numel() * 8 * 10 / (ev[0].elapsed_time(ev[1]) * 1e-3) / 1e9}
+    float(a.sum())
+    ev[0].record()
+    for _ in range(10):
+        a.sum()
+    ev[1].record()
+    torch.cuda.synchronize()
+    out['read_2GiB'] = {'GBs': a.numel() * 8 * 10 / (ev[0].elapsed_time(ev[1]) * 1e-3) / 1e9}
+    del a, b
+    torch.cuda.empty_cache()
+    log('device copy %.0f GB/s (read + write), read-only reduction %.0f GB/s'
+        % (out['copy_2GiB']['GBs'], out['read_2GiB']['GBs']))
+
+    def models():
+        yield 'C1 cantilever 24x48x24', cuboid_model(24, 48, 24, 1, precision='mixed')
+        yield 'C2 MBB 10x60x10 super-element', cuboid_model(10, 60, 10, 2, super_element=True, precision='mixed')
+        with tempfile.TemporaryDirectory() as td:
+            mask = value_noise_mask(256, 128, 128, seed=0, octaves=4, period=32)
+            path = td + '/value_noise.TopVoxel'
+            write_topvoxel(path, mask, *bottom_clamp_top_patch(mask))
+            yield 'C3 value noise 256x128x128', load_topvoxel_model(path, precision='mixed')
+        yield 'C4 cantilever 96x192x96', cuboid_model(96, 192, 96, 1, precision='mixed')
+        yield 'C5 cantilever 128x256x128', cuboid_model(128, 256, 128, 1, precision='mixed')
+
+    for name, m in models():
+        s = Solver(m)
+        g = torch.Generator().manual_seed(2)
+        nE = m.num_elements * (64 if m.super_element else 1)
+        x = torch.rand(nE, generator=g, dtype=torch.float64)
+        s.set_modulus((1e-6 + x ** 3 * (1 - 1e-6)).cuda())
+        s.assemble()
+        torch.cuda.synchronize()
+        Ne, Nn = m.num_elements, m.num_nodes
+        row = {'elements': Ne, 'nodes': Nn}
+        variants = [('fp64', 'fp64', None, 8)] + ([] if m.super_element else
+                                                  [('fp32', 'fp32', 'walsh', 4), ('bf16', 'fp32', 'bf16', 4),
+                                                   ('bf16x3', 'fp32', 'bf16x3', 4)])
+        for key, prec, mode, sz in variants:
+            if mode:
+                s.set_tuning(kfine_mfma=mode)
+            us = s.time_kbyu(1, prec, reps) * 1e3
+            ba = kbyu_alg_bytes(Nn, Ne, sz, sz, sz)
+            row[key] = {'us': us, 'bytes_alg': ba, 'GBs_alg': ba / us / 1e3}
+        if not m.super_element:
+            s.set_tuning(kfine_mfma='walsh')
+        out[name] = row
+        log('%s: %s' % (name, ', '.join('%s %.1f us (%.0f GB/s alg)' % (k, v['us'], v['GBs_alg'])
+                                         for k, v in row.items() if isinstance(v, dict))))
+        del s, m
+        torch.cuda.empty_cache()
+    return out
+
+
 def mex_timing(shape, reps):
     """The six MEX entry points on the connectivity and transfer maps of the `shape` cantilever (the
     super-element assembly on the 10x60x10 MBB model, its own configuration), FP64, inputs resident on
@@ -325,6 +423,9 @@ def main():
     ap.add_argument('--coarse', choices=('step', 'persistent', 'streaming'), default=None,
                     help='coarsest solve: launched step by step, as one persistent launch, or persistent with '
                          "non-temporal factor loads (default: the library's default)")
+    ap.add_argument('--kbyu', type=int, default=0, metavar='REPS',
+                    help='only time the finest K*u at every configuration shape (REPS launches each) and the '
+                         'device copy / read bandwidths')
     ap.add_argument('--mex', type=int, default=0, metavar='REPS',
                     help='only time the six MEX entry points (REPS calls each) at the --size shapes')
     args = ap.parse_args()
@@ -337,6 +438,9 @@ def main():
         return
     if args.mex:
         print(json.dumps({'mex_ms_per_call': mex_timing(tuple(args.size), args.mex)}), flush=True)
+        return
+    if args.kbyu:
+        print(json.dumps({'kbyu_microbench': kbyu_microbench(args.kbyu)}), flush=True)
         return
     import torch
     shape = tuple(args.size)
@@ -393,6 +497,9 @@ def main():
         v['GBs_alg'] = (b64 if k.startswith('fp64') else b32) / v['avg_us'] / 1e3
     log('finest K*u in the timed steps: ' + ', '.join('%s %.1f us x%d (%.0f GB/s alg)'
                                                       % (k, v['avg_us'], v['launches'], v['GBs_alg']) for k, v in kst.items()))
+    copy_gbs = device_copy_gbs()
+    log('device copy (the practical HBM ceiling): %.0f GB/s; the FP64 PCG operator reaches %.2f of it'
+        % (copy_gbs, gbs64 / copy_gbs))
     cpu = None
     if not args.no_cpu:
         try:
@@ -432,7 +539,8 @@ def main():
                                'launches of the mixed solver in the timed steps', 'bound': 'hbm',
                      'achieved': gbs64, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': gbs64 / HBM_PEAK_GBS,
                      'traffic': measured_traffic(shape), 'bytes_alg': b64, 'avg_launch_us': us64, 'launches': kst['fp64_ku']['launches'],
-                     'dense_fp64_flop_frac': tf64 / FP64_PEAK_TFLOPS},
+                     'dense_fp64_flop_frac': tf64 / FP64_PEAK_TFLOPS,
+                     'copy_GBs': copy_gbs, 'frac_of_copy': gbs64 / copy_gbs},
         'cpu_baseline': cpu,
     }
     print(json.dumps(out), flush=True)

@@ -237,6 +237,10 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *   TG_TUNE_COARSE_PERSISTENT  1 = run the coarsest solve as one persistent launch whose workgroups pass
  *                         the recurrence vectors to each other directly (default; planes of at most 1024
  *                         DOFs, no graph requested), 0 = step by step
+ *   TG_TUNE_COARSE_POLLS  polls of an input (a device-coherent load each, a memory round trip apart)
+ *                         before a stage of the persistent coarsest solve gives up (default 131072,
+ *                         ~0.1 s); one give-up ends every other wait of the launch and makes the solve
+ *                         fail with TG_ERR_HIP.  Small values force that path in tests.
  *   TG_TUNE_COARSE_STREAMING  1 = the persistent coarsest solve reads its factors with non-temporal
  *                         (streaming) loads (default; 174 against 178-185 us at 128 x 256 x 128),
  *                         0 = ordinary loads
@@ -246,7 +250,7 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *                         3 = as 2 with each operand split into two BF16 parts (three products) */
 enum { TG_TUNE_KFINE_KERNEL = 1, TG_TUNE_KFINE_ZCHUNK = 2, TG_TUNE_TIME_KFINE = 3, TG_TUNE_KELEM_PARTS = 4,
        TG_TUNE_COARSE_GRAPH = 5, TG_TUNE_KFINE_MFMA = 6, TG_TUNE_COARSE_PERSISTENT = 7,
-       TG_TUNE_COARSE_STREAMING = 8 };
+       TG_TUNE_COARSE_STREAMING = 8, TG_TUNE_COARSE_POLLS = 9 };
 int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value);
 /* Damped-Jacobi weight of the V-cycle smoother (the reference's global weightFactorJacobi_,
  * TOP_GPU.m:397, 784, 809-811; default 0.6).  0 < omega < 2.  The smoother only reduces every error

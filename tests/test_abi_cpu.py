@@ -18,6 +18,18 @@ def declared_symbols():
     return sorted(set(re.findall(r'\b([A-Za-z_][A-Za-z0-9_]*)\s*\(', src)) - {'if', 'sizeof'})
 
 
+def test_python_constants_match_the_header():
+    """Every TG_* enumerator of include/topgpu.h that the Python binding names has the header's value
+    (status codes, classes, precisions, tuning keys)."""
+    src = re.sub(r'/\*.*?\*/', '', open(os.path.join(ROOT, 'include', 'topgpu.h')).read(), flags=re.S)
+    header = {k: int(v) for k, v in re.findall(r'\b(TG_[A-Z0-9_]+)\s*=\s*(\d+)', src)}
+    named = {k: getattr(lib_mod, k) for k in dir(lib_mod) if k.startswith('TG_') and isinstance(getattr(lib_mod, k), int)}
+    for k in ('TG_ERR_HIP', 'TG_TUNE_COARSE_PERSISTENT', 'TG_TUNE_COARSE_POLLS', 'TG_TUNE_KFINE_MFMA'):
+        assert k in header and k in named, k
+    wrong = {k: (v, header[k]) for k, v in named.items() if k in header and header[k] != v}
+    assert not wrong, wrong
+
+
 def test_every_declared_symbol_is_exported():
     L = lib_mod.lib()
     names = declared_symbols()

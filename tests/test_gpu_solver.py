@@ -291,6 +291,41 @@ def test_coarse_solve_persistent_alternating_right_hand_sides():
         s.set_tuning(coarse_persistent=True)
 
 
+def test_coarse_solve_persistent_give_up_is_an_error():
+    """A stage of the persistent coarsest solve that waits too long for its input gives up, every other
+    wait of the launch then ends at once, the launch completes, and the solve reports TG_ERR_HIP instead
+    of returning a result.  Forced from the host with a poll limit of 1 on the 96 x 192 x 96 cantilever
+    (25 planes: later stages wait several microseconds); afterwards the default limit gives the same
+    solution as before, bitwise."""
+    import torch
+    from top_gpu_on_matlab_amd import _lib
+    from top_gpu_on_matlab_amd import top_gpu as G
+    from top_gpu_on_matlab_amd._lib import TopGpuError
+    from top_gpu_on_matlab_amd.model import cuboid_model
+    from top_gpu_on_matlab_amd.solver import Solver
+    s = Solver(cuboid_model(96, 192, 96, 1, precision='fp64'))
+    rng = np.random.default_rng(9)
+    x = torch.from_numpy(0.12 + rng.uniform(-0.05, 0.05, s.model.num_elements)).cuda()
+    s.set_modulus(G.simp(x, 1e-6, 1.0, 3.0))
+    s.assemble()
+    r = torch.from_numpy(rng.standard_normal(s.ndof(s.num_levels))).cuda()
+    ref = s.coarse_solve(r).clone()
+    failed = 0
+    try:
+        s.set_tuning(coarse_polls=1)
+        for _ in range(5):
+            try:
+                s.coarse_solve(r)
+            except TopGpuError as e:
+                assert e.code == _lib.TG_ERR_HIP and 'timed out' in str(e), e
+                failed += 1
+                break
+    finally:
+        s.set_tuning(coarse_polls=1 << 17)
+    assert failed == 1
+    assert torch.equal(s.coarse_solve(r), ref)
+
+
 @pytest.mark.parametrize('super_element', [False, True])
 def test_mixed_vcycle_matches_fp64(super_element):
     """The FP32 V-cycle (FP32 smoother, transfers and per-element matrices, the latter packed

@@ -11,6 +11,7 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, 'libtopgpu.so')
 
 TG_OK = 0
+TG_ERR_ARGS, TG_ERR_TYPE, TG_ERR_SHAPE, TG_ERR_HIP, TG_ERR_STATE, TG_ERR_NUMERIC = 1, 2, 3, 4, 5, 6
 TG_SINGLE, TG_DOUBLE, TG_INT32, TG_UINT32 = 1, 2, 3, 4
 TG_PREC_FP64, TG_PREC_MIXED = 0, 1
 
@@ -109,7 +110,8 @@ SIGNATURES = {
 }
 TG_EW_MUL, TG_EW_DIV_FLOOR, TG_EW_SET_WHERE = 0, 1, 2
 (TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK, TG_TUNE_TIME_KFINE, TG_TUNE_KELEM_PARTS, TG_TUNE_COARSE_GRAPH,
- TG_TUNE_KFINE_MFMA, TG_TUNE_COARSE_PERSISTENT, TG_TUNE_COARSE_STREAMING) = 1, 2, 3, 4, 5, 6, 7, 8
+ TG_TUNE_KFINE_MFMA, TG_TUNE_COARSE_PERSISTENT, TG_TUNE_COARSE_STREAMING, TG_TUNE_COARSE_POLLS) = \
+    1, 2, 3, 4, 5, 6, 7, 8, 9
 TG_RED_SUM, TG_RED_DOT, TG_RED_MAXABSDIFF, TG_RED_SHARPNESS = 0, 1, 2, 3
 
 _lib = None

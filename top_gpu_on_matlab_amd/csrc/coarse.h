@@ -56,6 +56,7 @@ struct CoarseSolver {
     // wide for it (more than 1024 DOFs per plane)
     bool use_persistent = true;
     bool pers_streaming = true;  // non-temporal loads of the factor rows
+    unsigned pers_polls = 1u << 17;  // polls of an input (~1 us apart) before a stage gives up (~0.1 s)
     bool pers_built = false, pers_ok = false;
     int pers_nk = 0, pers_groups = 0, pers_stages = 0, pers_L1 = 0, pers_L2 = 0;
     DevBuf<unsigned char> pers_table;        // the stage table (device copy)

@@ -501,7 +501,6 @@ struct PStage {
 };
 
 constexpr unsigned long long PERS_EMPTY = 0x7ff4c0a5e5e17e11ull;  // signalling NaN: "not written yet"
-constexpr unsigned PERS_POLLS = 1u << 22;                         // ~0.1 s of polling before giving up
 
 // The table's pointers are generic; these name them as global memory, so that the loads are global
 // (not flat) accesses.
@@ -516,12 +515,16 @@ __device__ __forceinline__ void coh_store(double *p, double v) {
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 // The value at p once an earlier stage has written it (bounded wait; the error flag records a give-up).
-__device__ __forceinline__ double await_value(const double *p, unsigned long long bits, int *err) {
+// A poll waits for its own load, so polls are about a memory round trip (~1 us) apart.  Every 64 polls
+// the error flag is read too: once any wait of the launch has given up, every other one ends at once.
+__device__ __forceinline__ double await_value(const double *p, unsigned long long bits, int *err, unsigned max_polls) {
     unsigned polls = 0;
     while (bits == PERS_EMPTY) {
         __builtin_amdgcn_s_sleep(1);
         bits = coh_bits(p);
-        if (++polls > PERS_POLLS) {
+        ++polls;
+        if (polls > max_polls ||
+            ((polls & 63) == 0 && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
             __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             bits = 0;
         }
@@ -534,7 +537,7 @@ __device__ __forceinline__ double await_value(const double *p, unsigned long lon
 // are written by earlier stages of this launch (polled until written).
 template <bool AWAIT>
 __device__ __forceinline__ void stage_vec(double *dst, int n, const double *a, int na, const double *b, int nb,
-                                          int *err) {
+                                          int *err, unsigned max_polls) {
     for (int j0 = 0; j0 < n; j0 += 8 * 256) {
         unsigned long long v[8];
 #pragma unroll
@@ -550,7 +553,7 @@ __device__ __forceinline__ void stage_vec(double *dst, int n, const double *a, i
             const int j = j0 + 256 * k + (int)threadIdx.x;
             if (j < n) {
                 double x = __longlong_as_double((long long)v[k]);
-                if (AWAIT && v[k] == PERS_EMPTY) x = await_value(j < na ? a + j : b + (j - na), v[k], err);
+                if (AWAIT && v[k] == PERS_EMPTY) x = await_value(j < na ? a + j : b + (j - na), v[k], err, max_polls);
                 dst[j] = x;
             }
         }
@@ -559,7 +562,7 @@ __device__ __forceinline__ void stage_vec(double *dst, int n, const double *a, i
 
 template <int NK, bool NT>
 __global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restrict__ stages, int nstages, int L1,
-                                                           int L2, int *err) {
+                                                           int L2, int *err, unsigned max_polls) {
     extern __shared__ double sv[];  // v1 staged in [0, L1), [v2a; v2b] in [L1, L1 + L2), zero past the ends
     constexpr int CH = 64 * NK;
     double *sv1 = sv, *sv2 = sv + L1;
@@ -590,7 +593,7 @@ __global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restr
         const PRow &q = S.q[c.qi];
         c.any = c.r - w < q.n;
         c.t1 = 0.0;
-        if (c.any && q.M1) stage_vec<false>(sv1, chunk_len(q.ld1), q.v1, q.n1, q.v1, 0, err);
+        if (c.any && q.M1) stage_vec<false>(sv1, chunk_len(q.ld1), q.v1, q.n1, q.v1, 0, err, max_polls);
         lds_barrier();  // unconditional: it also orders the previous part's sv2 reads before the next writes
         if (c.r < q.n) {
             if (q.M1) {
@@ -612,8 +615,8 @@ __global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restr
         double cval = 0.0;
         unsigned long long cbits = 0;
         if (q.c && lane == 0 && r < q.n) cbits = coh_bits(q.c + r);  // in the same round trip as the staging
-        if (c.any && q.M2) stage_vec<true>(sv2, chunk_len(q.ld2), q.v2a, q.n2a, q.v2b, q.n2b, err);
-        if (q.c && lane == 0 && r < q.n) cval = await_value(q.c + r, cbits, err);
+        if (c.any && q.M2) stage_vec<true>(sv2, chunk_len(q.ld2), q.v2a, q.n2a, q.v2b, q.n2b, err, max_polls);
+        if (q.c && lane == 0 && r < q.n) cval = await_value(q.c + r, cbits, err, max_polls);
         lds_barrier();  // unconditional, like part A's
         if (r < q.n) {
             double t2 = 0.0;
@@ -1049,9 +1052,11 @@ void CoarseSolver::enqueue_persistent(hipStream_t st) {
 #define TG_PERS_CASE(K)                                                                                   \
     case K:                                                                                               \
         if (pers_streaming)                                                                                \
-            k_coarse_persistent<K, true><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p); \
+            k_coarse_persistent<K, true><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p, \
+                                                                        pers_polls);                        \
         else                                                                                              \
-            k_coarse_persistent<K, false><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p); \
+            k_coarse_persistent<K, false><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p, \
+                                                                         pers_polls);                       \
         break;
         TG_PERS_CASE(1) TG_PERS_CASE(2) TG_PERS_CASE(3) TG_PERS_CASE(4) TG_PERS_CASE(5) TG_PERS_CASE(6)
         TG_PERS_CASE(7) TG_PERS_CASE(8) TG_PERS_CASE(9) TG_PERS_CASE(10) TG_PERS_CASE(11) TG_PERS_CASE(12)

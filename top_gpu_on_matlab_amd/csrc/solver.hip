@@ -1075,6 +1075,7 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
         else if (key == TG_TUNE_COARSE_GRAPH && (value == 0 || value == 1)) S.coarse.use_graph = value != 0;
         else if (key == TG_TUNE_COARSE_PERSISTENT && (value == 0 || value == 1)) S.coarse.use_persistent = value != 0;
         else if (key == TG_TUNE_COARSE_STREAMING && (value == 0 || value == 1)) S.coarse.pers_streaming = value != 0;
+        else if (key == TG_TUNE_COARSE_POLLS && value >= 1) S.coarse.pers_polls = (unsigned)value;
         else if (key == TG_TUNE_KFINE_MFMA && value >= 0 && value <= 3) S.kfine_mfma = value;
         else if (key == TG_TUNE_TIME_KFINE && value >= 0 && value <= 2) {
             S.time_fine = value;

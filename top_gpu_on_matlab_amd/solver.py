@@ -122,7 +122,7 @@ class Solver:
         return ms.value
 
     def set_tuning(self, kfine_kernel=None, kfine_zchunk=None, kelem_parts=None, coarse_graph=None, kfine_mfma=None,
-                   coarse_persistent=None, coarse_streaming=None):
+                   coarse_persistent=None, coarse_streaming=None, coarse_polls=None):
         """Finest-level K*u launch choice: kernel 'tiled' (default) or 'gather'; z planes per tile;
         threads per element of the FP64 per-element-matrix K*u; coarsest solve as one persistent launch
         (True, default; coarse_streaming=True, the default, reads its factors with non-temporal loads) or step by step,
@@ -133,6 +133,9 @@ class Solver:
         parts, three products)."""
         from ._lib import (TG_TUNE_COARSE_GRAPH, TG_TUNE_COARSE_PERSISTENT, TG_TUNE_COARSE_STREAMING,
                            TG_TUNE_KELEM_PARTS, TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_MFMA, TG_TUNE_KFINE_ZCHUNK)
+        if coarse_polls is not None:
+            from ._lib import TG_TUNE_COARSE_POLLS
+            check(lib().tg_solver_set_tuning(self.h, TG_TUNE_COARSE_POLLS, int(coarse_polls)))
         if coarse_streaming is not None:
             check(lib().tg_solver_set_tuning(self.h, TG_TUNE_COARSE_STREAMING, int(bool(coarse_streaming))))
         if coarse_persistent is not None:

@@ -13,11 +13,13 @@
 // matrix cores); the two chains' planes of a step share every launch.  The solve operators are
 //   forward  F_p = S_p^-1 A_{p,p-1} (chain A), S_q^-1 A_{q,q+1} (chain B), [S_m^-1 A_{m,m-1} | S_m^-1 A_{m,m+1}]
 //   backward B_p = N_p
-// and a solve is c = blockdiag(S_p^-1) b (all planes in one launch), the forward recurrences
-// z_p = c_p - F_p z_{p-1|p+1} of both chains advanced together (one launch per step), the separator
-// x_m = c_m - F_m [z_{m-1}; z_{m+1}], then x_p = z_p - B_p x_{p+1|p-1} outwards.  Everything is FP64
-// and deterministic (fixed summation orders, no atomics).  The solve sequence reads only buffers fixed
-// at setup, so it can also be captured once as a HIP graph and replayed.
+// and a solve is c = blockdiag(S_p^-1) b, the forward recurrences z_p = c_p - F_p z_{p-1|p+1} of both
+// chains advanced together, the separator x_m = c_m - F_m [z_{m-1}; z_{m+1}], then
+// x_p = z_p - B_p x_{p+1|p-1} outwards.  By default the whole sequence is one persistent launch whose
+// workgroups pass the recurrence vectors to each other (k_coarse_persistent, in coarse.hip); step by
+// step, it is one launch for c and one per step, and that sequence reads only buffers fixed at setup,
+// so it can also be captured once as a HIP graph and replayed.  Everything is FP64 and deterministic
+// (fixed summation orders, no floating-point atomics).
 #pragma once
 
 #include "kernels.h"

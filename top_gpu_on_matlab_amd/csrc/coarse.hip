@@ -479,9 +479,10 @@ void launch_chain(const ChainProb &A, const ChainProb &B, hipStream_t s) {
 // workgroup that needs an earlier stage's vector polls exactly the entries it stages until they are no
 // longer the sentinel.  Stores and these loads are device-scope atomic accesses, coherent across the
 // XCDs' separate L2 caches.  (A grid barrier between stages measured 2.6 us with per-group counters and
-// 8 us with one counter, on top of the exchange itself.)  Each workgroup runs ahead as far as its inputs
-// allow: the part of a stage that does not depend on earlier stages (staging the right-hand-side plane,
-// the M1 v1 partial sums, the first chunk of the M2 row) is done before it polls.  A poll that waits
+// 8 us with one counter, on top of the exchange itself.)  Each wavefront runs ahead as far as its inputs
+// allow: the loads of a stage that do not depend on earlier stages (its M1 row with the lane's entries
+// of the right-hand-side plane, the first chunk of its M2 row) are issued before it polls the previous
+// stage's input, so a stage costs one memory round trip.  A poll that waits
 // longer than a bounded number of tries sets an error flag and gives up, so the launch always ends;
 // the host checks the flag.  G is at most the number of workgroups the device holds at once, so every
 // producer is resident while consumers wait.
@@ -561,11 +562,10 @@ __device__ __forceinline__ void stage_vec(double *dst, int n, const double *a, i
 }
 
 template <int NK, bool NT>
-__global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restrict__ stages, int nstages, int L1,
-                                                           int L2, int *err, unsigned max_polls) {
-    extern __shared__ double sv[];  // v1 staged in [0, L1), [v2a; v2b] in [L1, L1 + L2), zero past the ends
+__global__ __launch_bounds__(256, 2) void k_coarse_persistent(const PStage *__restrict__ stages, int nstages, int L2,
+                                                              int *err, unsigned max_polls) {
+    extern __shared__ double sv[];  // two buffers of L2: [v2a; v2b] of stage s in buffer s & 1, zero past the ends
     constexpr int CH = 64 * NK;
-    double *sv1 = sv, *sv2 = sv + L1;
     const int g = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     auto chunk_len = [&](int64_t ld) { return (int)(((ld + CH - 1) / CH) * CH); };
     // NT: the factor rows are read once per solve; streaming (non-temporal) loads keep them from
@@ -578,78 +578,74 @@ __global__ __launch_bounds__(256) void k_coarse_persistent(const PStage *__restr
                                       : 0.0;
     };
     // A stage in flight: this wavefront's problem and row (one row per wavefront), whether the
-    // workgroup has rows at all, the M1 v1 partial sums and the first M2 chunk.
+    // workgroup has rows at all, and the stage's operands that depend on no earlier stage, in
+    // registers: the M1 row with this lane's entries of v1 (ld1 <= CH: one chunk) and the first M2 chunk.
     struct Cur {
         int qi, r;
         bool any;
-        double t1;
-        double pre2[NK];
+        double m1[NK], v1[NK], m2[NK];
     };
-    // Part A of stage s, which depends on no earlier stage: v1, the M1 v1 partial sums, the first M2 chunk.
+    // Part A of stage s: its loads, issued without waiting for any of them.
     auto part_a = [&](int s, Cur &c) {
         const PStage &S = stages[s];
         c.qi = (S.nq == 2 && g >= S.gA) ? 1 : 0;
         c.r = 4 * (c.qi ? g - S.gA : g) + w;
         const PRow &q = S.q[c.qi];
         c.any = c.r - w < q.n;
-        c.t1 = 0.0;
-        if (c.any && q.M1) stage_vec<false>(sv1, chunk_len(q.ld1), q.v1, q.n1, q.v1, 0, err, max_polls);
-        lds_barrier();  // unconditional: it also orders the previous part's sv2 reads before the next writes
         if (c.r < q.n) {
             if (q.M1) {
-                const double *row = q.M1 + (int64_t)c.r * q.ld1;
-                for (int64_t c0 = 0; c0 < q.ld1; c0 += CH) {
-                    double a[NK];
-                    load_chunk(row, q.ld1, c0, a);
+                load_chunk(q.M1 + (int64_t)c.r * q.ld1, q.ld1, 0, c.m1);
 #pragma unroll
-                    for (int k = 0; k < NK; ++k) c.t1 = fma(a[k], sv1[c0 + 64 * k + lane], c.t1);
-                }
+                for (int k = 0; k < NK; ++k) c.v1[k] = 64 * k + lane < q.n1 ? gmem(q.v1)[64 * k + lane] : 0.0;
             }
-            if (q.M2) load_chunk(q.M2 + (int64_t)c.r * q.ld2, q.ld2, 0, c.pre2);
+            if (q.M2) load_chunk(q.M2 + (int64_t)c.r * q.ld2, q.ld2, 0, c.m2);
         }
     };
-    // Part B: the earlier stages' vectors, the M2 product and the store.
+    // Part B: the earlier stages' vectors, the products and the store.
     auto part_b = [&](int s, Cur &c) {
         const PRow &q = stages[s].q[c.qi];
         const int r = c.r;
+        double *sv2 = sv + (s & 1) * L2;
         double cval = 0.0;
         unsigned long long cbits = 0;
         if (q.c && lane == 0 && r < q.n) cbits = coh_bits(q.c + r);  // in the same round trip as the staging
         if (c.any && q.M2) stage_vec<true>(sv2, chunk_len(q.ld2), q.v2a, q.n2a, q.v2b, q.n2b, err, max_polls);
         if (q.c && lane == 0 && r < q.n) cval = await_value(q.c + r, cbits, err, max_polls);
-        lds_barrier();  // unconditional, like part A's
+        lds_barrier();  // unconditional (also in workgroups without rows in the stage)
         if (r < q.n) {
-            double t2 = 0.0;
-            if (q.M2) {
-                const double *row = q.M2 + (int64_t)r * q.ld2;
-                for (int64_t c0 = 0; c0 < q.ld2; c0 += CH) {
-                    double a[NK];
-                    if (c0 == 0) {
+            double t1 = 0.0, t2 = 0.0;
+            if (q.M1) {
 #pragma unroll
-                        for (int k = 0; k < NK; ++k) a[k] = c.pre2[k];
-                    } else {
-                        load_chunk(row, q.ld2, c0, a);
-                    }
+                for (int k = 0; k < NK; ++k) t1 = fma(c.m1[k], c.v1[k], t1);
+            }
+            if (q.M2) {
+#pragma unroll
+                for (int k = 0; k < NK; ++k) t2 = fma(c.m2[k], sv2[64 * k + lane], t2);
+                const double *row = q.M2 + (int64_t)r * q.ld2;
+                for (int64_t c0 = CH; c0 < q.ld2; c0 += CH) {
+                    double a[NK];
+                    load_chunk(row, q.ld2, c0, a);
 #pragma unroll
                     for (int k = 0; k < NK; ++k) t2 = fma(a[k], sv2[c0 + 64 * k + lane], t2);
                 }
             }
-            double t = c.t1 - t2;
+            double t = t1 - t2;
             for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
             if (lane == 0) coh_store(q.y + r, cval + t);
         }
     };
     // Two stages in flight: part A of stage s + 1 (its loads) is issued before part B of stage s waits
-    // for its input.  LDS: part A writes only sv1 and part B only sv2, each between the other part's
-    // barrier and its own, so every write follows every wave's reads of the same region one part earlier
-    // (the barriers are taken also by workgroups without rows in a stage).
-    Cur cur, nxt;
-    part_a(0, cur);
-    lds_barrier();  // stage 0's sv1 reads before stage 1's sv1 writes (no part B lies between them)
-    for (int s = 0; s < nstages; ++s) {
-        if (s + 1 < nstages) part_a(s + 1, nxt);
-        part_b(s, cur);
-        cur = nxt;
+    // for its input; two register sets in turn.  LDS: part B of stage s writes buffer s & 1 before its
+    // barrier and reads it after; the next write of that buffer (stage s + 2) follows the barrier of
+    // stage s + 1, which every wave reaches after its reads of stage s.
+    Cur ca, cb;
+    part_a(0, ca);
+    for (int s = 0; s < nstages; s += 2) {
+        if (s + 1 < nstages) part_a(s + 1, cb);
+        part_b(s, ca);
+        if (s + 1 == nstages) break;
+        if (s + 2 < nstages) part_a(s + 2, ca);
+        part_b(s + 1, cb);
     }
 }
 
@@ -1002,8 +998,9 @@ void CoarseSolver::build_persistent(hipStream_t st) {
     if (tab.empty()) return;
     const int64_t ch = 64 * (int64_t)nk;
     const int64_t L1 = ((std::max<int64_t>(maxld1, 1) + ch - 1) / ch) * ch;
+    if (L1 > ch) return;  // the kernel keeps an M1 row in one chunk of registers
     const int64_t L2 = ((std::max<int64_t>(maxld2, 1) + ch - 1) / ch) * ch;
-    const size_t lds = (size_t)(L1 + L2) * sizeof(double);
+    const size_t lds = 2 * (size_t)L2 * sizeof(double);  // two stage buffers
     if (lds > 64 * 1024) return;
     int dev = 0, cus = 0, per_cu = 0;
     TG_HIP(hipGetDevice(&dev), id);
@@ -1046,16 +1043,16 @@ void CoarseSolver::build_persistent(hipStream_t st) {
 
 void CoarseSolver::enqueue_persistent(hipStream_t st) {
     const char *id = "tg_solver:coarse";
-    const size_t lds = (size_t)(pers_L1 + pers_L2) * sizeof(double);
+    const size_t lds = 2 * (size_t)pers_L2 * sizeof(double);
     const PStage *tab = (const PStage *)pers_table.p;
     switch (pers_nk) {
 #define TG_PERS_CASE(K)                                                                                   \
     case K:                                                                                               \
         if (pers_streaming)                                                                                \
-            k_coarse_persistent<K, true><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p, \
+            k_coarse_persistent<K, true><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L2, pers_err.p, \
                                                                         pers_polls);                        \
         else                                                                                              \
-            k_coarse_persistent<K, false><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L1, pers_L2, pers_err.p, \
+            k_coarse_persistent<K, false><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L2, pers_err.p, \
                                                                          pers_polls);                       \
         break;
         TG_PERS_CASE(1) TG_PERS_CASE(2) TG_PERS_CASE(3) TG_PERS_CASE(4) TG_PERS_CASE(5) TG_PERS_CASE(6)

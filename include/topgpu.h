@@ -230,8 +230,10 @@ int tg_solver_time_kbyu(tg_solver *s, int32_t level, int32_t precision, int32_t 
  *   TG_TUNE_TIME_KFINE    1 = bracket every tiled finest K*u launch with HIP events, 2 = only the
  *                         plain products (the PCG operator), 0 = stop; 1 and 2 clear the record
  *   TG_TUNE_KELEM_PARTS   threads per element of the FP64 per-element-matrix K*u (coarse levels and
- *                         the super-element finest level): 4, 6, 8, 12 or 24 (default 24); the FP32
- *                         V-cycle's packed symmetric product always uses one thread per element
+ *                         the super-element finest level): 1 (default) = the packed upper triangles of
+ *                         the symmetric element matrices, 300 of 576 entries; 4, 6, 8, 12 or 24 = the
+ *                         full matrices, split over that many threads; the FP32 V-cycle's product is
+ *                         always the packed one
  *   TG_TUNE_COARSE_GRAPH  1 = replay the coarsest solve's step-by-step launch sequence as a captured
  *                         HIP graph, 0 = launch it kernel by kernel (measured 1% faster per PCG iteration)
  *   TG_TUNE_COARSE_PERSISTENT  1 = run the coarsest solve as one persistent launch whose workgroups pass

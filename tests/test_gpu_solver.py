@@ -75,6 +75,28 @@ def test_kbyu_all_levels(pair):
         assert _rel(got, ref) < 1e-13, (name, level, _rel(got, ref))
 
 
+def test_kbyu_element_matrix_forms_agree(pair):
+    """The FP64 per-element-matrix K*u (coarse levels, and the finest level of a super-element model)
+    from the packed upper triangles (the default) and from the full matrices split over 24 and 4
+    threads per element: each within 1e-13 of the oracle, the two splits bitwise equal (each output
+    is one thread's sum in the same order)."""
+    name, om, osv, gm, gsv, E = pair
+    rng = np.random.default_rng(5)
+    try:
+        for level in range(1, om.numLevels + 1):
+            n = 3 * om.levels[level - 1].numNodes
+            u = rng.standard_normal(n)
+            ref = osv.kbyu(u, level)
+            got = {}
+            for parts in (1, 24, 4):
+                gsv.set_tuning(kelem_parts=parts)
+                got[parts] = gsv.kbyu(u, level).cpu().numpy()
+                assert _rel(got[parts], ref) < 1e-13, (name, level, parts, _rel(got[parts], ref))
+            assert np.array_equal(got[24], got[4]), (name, level)
+    finally:
+        gsv.set_tuning(kelem_parts=1)
+
+
 @pytest.mark.parametrize('nu', [0.3, 0.27])
 def test_finest_kbyu_kernels_agree(nu):
     """The LDS-tiled Walsh-basis K*u (compile-time unit matrix for nu = 0.3, runtime matrix

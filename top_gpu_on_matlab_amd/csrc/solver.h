@@ -18,7 +18,8 @@ struct DevLevel {
     DevBuf<double> diag64, r64, x64, t64;  // Jacobi diagonal and V-cycle vectors (full grid, interleaved)
     DevBuf<float> diag32, r32, x32, t32;
     DevBuf<double> Ks64;                   // element matrices, entry-major [576 x ldk]
-    DevBuf<float> Ks32;
+    DevBuf<double> Ks64p;                  // FP64 mode: their packed upper triangles [300 x ldk] (K*u)
+    DevBuf<float> Ks32;                    // mixed mode: the same in FP32
     DevBuf<double> fv64;                   // per-element K*u products, entry-major [24 x ldk]
     DevBuf<float> fv32;
     DevBuf<double> coeffF64;               // transferMatCoeffi on the finer grid's nodes (levels >= 2)
@@ -50,7 +51,7 @@ struct Solver {
     hipEvent_t *timing_slot(int key);
     int kfine_zchunk = 12;  // node planes per workgroup of the tiled kernel
     int kfine_mfma = 0;     // FP32 tiled finest K*u: 0 Walsh form, 1 / 2 element products on MFMA (FP32 / BF16)
-    int kelem_parts = 24;   // threads per element in the per-element-matrix K*u
+    int kelem_parts = 1;    // threads per element in the FP64 per-element-matrix K*u (1: packed symmetric)
     bool has_modulus = false, assembled = false;
     std::vector<DevLevel> lv;
     CoarseSolver coarse;

@@ -301,6 +301,8 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
             if (prec == TG_PREC_MIXED) {
                 D.Ks32.alloc(KSYM_N * D.ldk);  // packed symmetric (FP32 V-cycle)
                 D.fv32.alloc(24 * D.ldk);
+            } else if (i < L - 1) {
+                D.Ks64p.alloc(KSYM_N * D.ldk);  // packed symmetric (FP64 V-cycle; the coarsest level has no K*u)
             }
             // transfer coefficients on the finer grid (1 where a node is unused)
             const HostLevel &F = m.lv[i - 1];
@@ -396,6 +398,8 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
         if (prec == TG_PREC_MIXED) {
             F.Ks32.alloc(KSYM_N * F.ldk);  // packed symmetric (FP32 V-cycle)
             F.fv32.alloc(24 * F.ldk);
+        } else {
+            F.Ks64p.alloc(KSYM_N * F.ldk);  // packed symmetric (FP64 V-cycle and PCG operator)
         }
     } else {
         E64.alloc(nEf);
@@ -518,6 +522,12 @@ void Solver::assemble() {
         }
         if (!super) k_convert<double, float><<<grid_for(F.g.elems(), 256), 256, 0, st>>>(E64, E32, F.g.elems());
         TG_LAUNCH_CHECK(id);
+    } else {
+        for (int i = 0; i < L - 1; ++i) {
+            DevLevel &D = lv[i];
+            if (D.Ks64p.n) k_pack_sym<double, double><<<grid_for(KSYM_N * D.ldk, 256), 256, 0, st>>>(D.Ks64, D.Ks64p, D.ldk);
+        }
+        TG_LAUNCH_CHECK(id);
     }
     assembled = true;
 }
@@ -525,6 +535,7 @@ void Solver::assemble() {
 template <typename T, bool MASKED>
 static void launch_kelem_vec(int parts, hipStream_t st, const T *u, T *fv, const T *Km, int64_t ldk,
                              const uint8_t *eleMask, const uint8_t *nodeMask, const Grid &g) {
+    if (parts != 4 && parts != 6 && parts != 8 && parts != 12) parts = 24;  // the grid must match the variant
     const unsigned ge = grid_for(parts * ldk, 256);
     switch (parts) {
         case 4: k_kelem_vec<T, MASKED, 4><<<ge, 256, 0, st>>>(u, fv, Km, ldk, eleMask, nodeMask, g); break;
@@ -654,17 +665,20 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
             else k_kfine<T, EPI_JACOBI, false><<<gr, 256, 0, st>>>(u, out, E, D.nodeMask, K0, (T)1, D.g, rr, dg, w);
         }
     } else {
-        const T *Km = f64 ? (const T *)D.Ks64.p : (const T *)D.Ks32.p;
+        // FP64: the packed upper triangles (one thread per element) unless a thread split of the full
+        // matrices is asked for; FP32: always packed
+        const bool packed = !f64 || (kelem_parts == 1 && D.Ks64p.n);
+        const T *Km = f64 ? (const T *)(packed ? D.Ks64p.p : D.Ks64.p) : (const T *)D.Ks32.p;
         T *fv = f64 ? (T *)D.fv64.p : (T *)D.fv32.p;
         const unsigned gs = grid_for(D.g.elems(), 256);
         if (i == 0) {
-            if constexpr (sizeof(T) == 8) launch_kelem_vec<T, true>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            if (!packed) launch_kelem_vec<T, true>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             else k_kelem_sym<T, true><<<gs, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else k_kelem_gather<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
         } else {
-            if constexpr (sizeof(T) == 8) launch_kelem_vec<T, false>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
+            if (!packed) launch_kelem_vec<T, false>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             else k_kelem_sym<T, false><<<gs, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
             else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
@@ -1070,7 +1084,7 @@ int tg_solver_set_tuning(tg_solver *s, int32_t key, int32_t value) {
         Solver &S = S_of(s, id);
         if (key == TG_TUNE_KFINE_KERNEL && (value == 0 || value == 1)) S.kfine_kernel = value;
         else if (key == TG_TUNE_KFINE_ZCHUNK && value >= 1 && value <= 4096) S.kfine_zchunk = value;
-        else if (key == TG_TUNE_KELEM_PARTS && (value == 4 || value == 6 || value == 8 || value == 12 || value == 24))
+        else if (key == TG_TUNE_KELEM_PARTS && (value == 1 || value == 4 || value == 6 || value == 8 || value == 12 || value == 24))
             S.kelem_parts = value;
         else if (key == TG_TUNE_COARSE_GRAPH && (value == 0 || value == 1)) S.coarse.use_graph = value != 0;
         else if (key == TG_TUNE_COARSE_PERSISTENT && (value == 0 || value == 1)) S.coarse.use_persistent = value != 0;

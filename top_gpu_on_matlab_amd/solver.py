@@ -124,7 +124,8 @@ class Solver:
     def set_tuning(self, kfine_kernel=None, kfine_zchunk=None, kelem_parts=None, coarse_graph=None, kfine_mfma=None,
                    coarse_persistent=None, coarse_streaming=None, coarse_polls=None):
         """Finest-level K*u launch choice: kernel 'tiled' (default) or 'gather'; z planes per tile;
-        threads per element of the FP64 per-element-matrix K*u; coarsest solve as one persistent launch
+        threads per element of the FP64 per-element-matrix K*u (1, the default: the packed upper triangles,
+        one thread per element; 4, 6, 8, 12 or 24: the full matrices split over that many threads); coarsest solve as one persistent launch
         (True, default; coarse_streaming=True, the default, reads its factors with non-temporal loads) or step by step,
         then either replayed as a captured HIP graph (coarse_graph=True)
         or launched kernel by kernel; the mixed V-cycle's finest element products: 'walsh' (the Walsh

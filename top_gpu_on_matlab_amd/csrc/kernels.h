@@ -258,11 +258,14 @@ __global__ __launch_bounds__(256) void k_kelem_gather(const T *__restrict__ f, i
                                                       const T *__restrict__ u, const uint8_t *__restrict__ eleMask,
                                                       const uint8_t *__restrict__ mask, const Grid g,
                                                       const T *__restrict__ r, const T *__restrict__ diag, T omega) {
-    const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= g.nodes()) return;
-    const int iy = (int)(n % g.NY);
-    const int64_t t = n / g.NY;
-    const int ix = (int)(t % g.NX), iz = (int)(t / g.NX);
+    // one thread per node of node plane blockIdx.y; in-plane and element indices in 32 bits (the host
+    // checks that they fit), neighbour elements by added strides
+    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int nYX = g.NY * g.NX;
+    if (p >= nYX) return;
+    const int iz = (int)blockIdx.y;
+    const int ix = p / g.NY, iy = p - ix * g.NY;
+    const int64_t n = p + (int64_t)nYX * iz;
     const uint8_t mk = mask[n];
     if (!(mk & NODE_ACTIVE)) {
         out[3 * n] = 0;
@@ -271,6 +274,8 @@ __global__ __launch_bounds__(256) void k_kelem_gather(const T *__restrict__ f, i
         return;
     }
     T acc[3] = {0, 0, 0};
+    const int eYX = g.EY * g.EX;
+    const int e00 = iy + g.EY * ix + eYX * iz;  // element (iy, ix, iz) (meaningful only where in the grid)
 #pragma unroll
     for (int oz = -1; oz <= 0; ++oz)
 #pragma unroll
@@ -280,7 +285,7 @@ __global__ __launch_bounds__(256) void k_kelem_gather(const T *__restrict__ f, i
                 const int ey = iy + oy, ex = ix + ox, ez = iz + oz;
                 const bool ok = (unsigned)ey < (unsigned)g.EY && (unsigned)ex < (unsigned)g.EX &&
                                 (unsigned)ez < (unsigned)g.EZ;
-                const int64_t e = ok ? ey + (int64_t)g.EY * (ex + (int64_t)g.EX * ez) : 0;
+                const int e = e00 + oy + (ox ? -g.EY : 0) + (oz ? -eYX : 0);
                 if (ok && eleMask[e]) {
                     const int a = vtx_of(-oy, -ox, -oz);
                     acc[0] += f[(int64_t)(3 * a) * ldk + e];
@@ -400,7 +405,10 @@ __global__ __launch_bounds__(RT_THREADS) void k_restrict_tiled(const T *__restri
     const int ly = el % RT_E, lx = el / RT_E;
     const int ey = e0y + ly, ex = e0x + lx;
     const bool own = ey < gc.EY && ex < gc.EX;
-    T pre[NI];
+    // the next plane's raw values (the division waits for them, so it is done at the LDS store, a
+    // plane later, and the loads stay in flight meanwhile); out-of-grid slots read node 0 and store 0
+    T preR[NI], preC[NI];
+    bool preOk[NI];
     auto fetch = [&](int iz) {
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
@@ -409,8 +417,9 @@ __global__ __launch_bounds__(RT_THREADS) void k_restrict_tiled(const T *__restri
             const int gy = S * e0y + node % FN, gx = S * e0x + node / FN;
             const bool ok = gy < gf.NY && gx < gf.NX && iz < gf.NZ;
             const int64_t n = ok ? gy + (int64_t)gf.NY * gx + NYX * iz : 0;
-            const T v = rf[3 * n + c] / coeff[n];
-            pre[k] = ok ? v : (T)0;
+            preR[k] = rf[3 * n + c];
+            preC[k] = coeff[n];
+            preOk[k] = ok;
         }
     };
     const T *pv = sV + oc * FN * FN + S * ly + FN * S * lx;
@@ -422,7 +431,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_restrict_tiled(const T *__restri
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
             const int i = t + RT_THREADS * k;
-            if (i < NV) sV[(i % 3) * FN * FN + i / 3] = pre[k];
+            if (i < NV) sV[(i % 3) * FN * FN + i / 3] = preOk[k] ? preR[k] / preC[k] : (T)0;
         }
         lds_barrier();
         if (iz < iz1) fetch(iz + 1);

@@ -532,6 +532,13 @@ void Solver::assemble() {
     assembled = true;
 }
 
+// Grid of k_kelem_gather: one row per node plane (its in-plane and element indices are 32-bit).
+static dim3 gather_grid(const Grid &g) {
+    if ((int64_t)g.NY * g.NX >= (1ll << 31) || g.elems() >= (1ll << 31) || g.NZ > 65535)
+        fail(TG_ERR_SHAPE, "tg_solver:kbyu", "grid too large for the element-gather kernel's indices");
+    return dim3(grid_for((int64_t)g.NY * g.NX, 256), (unsigned)g.NZ);
+}
+
 template <typename T, bool MASKED>
 static void launch_kelem_vec(int parts, hipStream_t st, const T *u, T *fv, const T *Km, int64_t ldk,
                              const uint8_t *eleMask, const uint8_t *nodeMask, const Grid &g) {
@@ -671,18 +678,19 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
         const T *Km = f64 ? (const T *)(packed ? D.Ks64p.p : D.Ks64.p) : (const T *)D.Ks32.p;
         T *fv = f64 ? (T *)D.fv64.p : (T *)D.fv32.p;
         const unsigned gs = grid_for(D.g.elems(), 256);
+        const dim3 grp = gather_grid(D.g);
         if (i == 0) {
             if (!packed) launch_kelem_vec<T, true>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             else k_kelem_sym<T, true><<<gs, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
-            if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
-            else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
-            else k_kelem_gather<T, EPI_JACOBI, true><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, true><<<grp, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, true><<<grp, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else k_kelem_gather<T, EPI_JACOBI, true><<<grp, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
         } else {
             if (!packed) launch_kelem_vec<T, false>(kelem_parts, st, u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
             else k_kelem_sym<T, false><<<gs, 256, 0, st>>>(u, fv, Km, D.ldk, D.eleMask, D.nodeMask, D.g);
-            if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
-            else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
-            else k_kelem_gather<T, EPI_JACOBI, false><<<gr, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            if (epi == EPI_KU) k_kelem_gather<T, EPI_KU, false><<<grp, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else if (epi == EPI_RESID) k_kelem_gather<T, EPI_RESID, false><<<grp, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
+            else k_kelem_gather<T, EPI_JACOBI, false><<<grp, 256, 0, st>>>(fv, D.ldk, out, u, D.eleMask, D.nodeMask, D.g, rr, dg, w);
         }
     }
     TG_LAUNCH_CHECK("tg_solver:kbyu");
@@ -698,7 +706,7 @@ void Solver::restrict_level(int c, const T *rf, T *rc) {
                                    ((C.g.EZ + RT_Z - 1) / RT_Z));
     if (C.span == 4) k_restrict_tiled<T, 4><<<gt, RT_THREADS, 0, st>>>(rf, cf, C.g, F.g, fv, C.ldk);
     else k_restrict_tiled<T, 2><<<gt, RT_THREADS, 0, st>>>(rf, cf, C.g, F.g, fv, C.ldk);
-    k_kelem_gather<T, EPI_KU, false><<<grid_for(C.g.nodes(), 256), 256, 0, st>>>(
+    k_kelem_gather<T, EPI_KU, false><<<gather_grid(C.g), 256, 0, st>>>(
         fv, C.ldk, rc, nullptr, C.eleMask, C.nodeMask, C.g, nullptr, nullptr, (T)0);
     TG_LAUNCH_CHECK("tg_solver:restrict");
 }

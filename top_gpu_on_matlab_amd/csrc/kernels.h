@@ -601,12 +601,18 @@ __global__ __launch_bounds__(RED_THREADS) void k_dot_partial(const T *__restrict
 }
 
 // out[slot] = sum(partial[0..nb))
+// host (optional): a device-mapped pinned host copy of out, written too, so the host reads the sum
+// without a copy
 static __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__restrict__ partial, int nb,
-                                                              double *__restrict__ out, int slot) {
+                                                              double *__restrict__ out, int slot,
+                                                              double *__restrict__ host = nullptr) {
     double s = 0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[i];
     s = block_sum<double>(s);
-    if (threadIdx.x == 0) out[slot] = s;
+    if (threadIdx.x == 0) {
+        out[slot] = s;
+        if (host) host[slot] = s;
+    }
 }
 
 }  // namespace tg

@@ -88,7 +88,8 @@ struct PcgWork {
     int64_t n = 0;
     hipStream_t st = nullptr;
     DevBuf<double> r, z, p, v, partial, scal;
-    double *h_scal = nullptr;
+    double *h_scal = nullptr;  // pinned host copies of the scalars the host reads (S_BB, S_RR) ...
+    double *d_hscal = nullptr;  // ... and their device address, written by the final reductions
     float *r32 = nullptr;     // optional FP32 copy of r kept by the update (mixed V-cycle input)
     bool r32_fresh = false;   // r32 holds the current r
     float *z32 = nullptr;     // optional FP32 preconditioner output (mixed V-cycle) ...
@@ -104,18 +105,23 @@ struct PcgWork {
         partial.alloc(RED_BLOCKS);
         scal.alloc(16);
         scal.zero(st);
-        if (!h_scal) TG_HIP(hipHostMalloc((void **)&h_scal, 16 * sizeof(double), hipHostMallocDefault), "tg_pcg:alloc");
+        if (!h_scal) {
+            TG_HIP(hipHostMalloc((void **)&h_scal, 16 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent),
+                   "tg_pcg:alloc");
+            TG_HIP(hipHostGetDevicePointer((void **)&d_hscal, h_scal, 0), "tg_pcg:alloc");
+        }
     }
     ~PcgWork() {
         if (h_scal) (void)hipHostFree(h_scal);
     }
     void dot(const double *a, const double *b, int slot) {
         k_dot_partial<double><<<RED_BLOCKS, RED_THREADS, 0, st>>>(a, b, n, partial);
-        k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, slot);
+        k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, slot, d_hscal);
         TG_LAUNCH_CHECK("tg_pcg:dot");
     }
+    // A scalar the stream's last final reduction into slot also left in pinned host memory (dot, or the
+    // residual norm of the update), once the stream has finished.
     double read(int slot) {
-        TG_HIP(hipMemcpyAsync(h_scal + slot, scal.p + slot, sizeof(double), hipMemcpyDeviceToHost, st), "tg_pcg");
         TG_HIP(hipStreamSynchronize(st), "tg_pcg");
         return h_scal[slot];
     }
@@ -160,7 +166,7 @@ struct PcgWork {
             } else {
                 k_pcg_update<false><<<RED_BLOCKS, RED_THREADS, 0, st>>>(r, v, n, scal, sx1, S_PV, partial, nullptr);
             }
-            k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR);
+            k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR, d_hscal);
             TG_LAUNCH_CHECK(id);
             res = std::sqrt(read(S_RR)) / normB;
             if (!std::isfinite(res)) fail(TG_ERR_NUMERIC, id, "non-finite residual at iteration %d", its);

@@ -12,17 +12,42 @@ namespace tg {
 
 enum { S_PV = 0, S_RR = 1, S_BB = 2, S_XA = 3, S_XB = 4 };
 
+// A dot left as partial sums for its consumer to finish (one launch fewer per dot).
+struct DeferredDot {
+    const double *parts = nullptr;
+    int n = 0, slot = -1;
+};
+
+// The sum of n partials in k_reduce_final's fixed order (the same threads, strides and tree: the same
+// value), in every thread of a RED_THREADS-wide workgroup.
+__device__ inline double sum_partials(const double *__restrict__ parts, int n) {
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s += parts[i];
+    s = block_sum<double>(s);
+    __syncthreads();  // every thread has read the sum before the shared array is used again
+    return s;
+}
+
 // r -= lambda v; partial sums of r.r  (TOP_GPU.m:631-632, 637).  The iterate's update y += lambda p
 // (:630) is applied later, by the next direction update, which reads p anyway (or by k_pcg_axpy when
 // the loop stops): the same operation on the same values, one pass over p and y fewer per iteration.
 // R32: also store r rounded to FP32 (the mixed V-cycle's input, TOP_GPU.m:775-777), saving a
-// conversion pass.
+// conversion pass.  pvp (optional): p.v as the operator's partial sums, finished here (workgroup 0
+// stores it in scal[ipv] for later launches).
 template <bool R32>
 __global__ __launch_bounds__(RED_THREADS) static void k_pcg_update(double *__restrict__ r, const double *__restrict__ v,
-                                                                   int64_t n, const double *__restrict__ scal, int ix1,
+                                                                   int64_t n, double *__restrict__ scal, int ix1,
                                                                    int ipv, double *__restrict__ partial,
-                                                                   float *__restrict__ r32) {
-    const double lam = scal[ix1] / scal[ipv];
+                                                                   float *__restrict__ r32, const double *__restrict__ pvp,
+                                                                   int npv) {
+    double pv;
+    if (pvp) {
+        pv = sum_partials(pvp, npv);
+        if (blockIdx.x == 0 && threadIdx.x == 0) scal[ipv] = pv;
+    } else {
+        pv = scal[ipv];
+    }
+    const double lam = scal[ix1] / pv;
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t lo = (int64_t)blockIdx.x * chunk, hi = lo + chunk < n ? lo + chunk : n;
     double s = 0;
@@ -37,7 +62,8 @@ __global__ __launch_bounds__(RED_THREADS) static void k_pcg_update(double *__res
 }
 
 // y += (x1 / pv) p, then p = z + (x2 / x1) p  (TOP_GPU.m:630, 652): the previous iteration's update of
-// the iterate and the new direction; Z = float: z as the mixed V-cycle left it (widening exact)
+// the iterate and the new direction; Z = float: z as the mixed V-cycle left it (widening exact).  One
+// thread per entry: a grid-stride form that could also finish the z.r dot measured slower (5 streams).
 template <typename Z>
 __global__ static void k_pcg_direction(double *__restrict__ p, double *__restrict__ y, const Z *__restrict__ z,
                                        int64_t n, const double *__restrict__ scal, int ix2, int ix1, int ipv) {
@@ -88,6 +114,8 @@ struct PcgWork {
     int64_t n = 0;
     hipStream_t st = nullptr;
     DevBuf<double> r, z, p, v, partial, scal;
+    bool defer_ok = false;      // the operator may leave p.v as partial sums ...
+    DeferredDot deferred;       // ... here, for the update to finish
     double *h_scal = nullptr;  // pinned host copies of the scalars the host reads (S_BB, S_RR) ...
     double *d_hscal = nullptr;  // ... and their device address, written by the final reductions
     float *r32 = nullptr;     // optional FP32 copy of r kept by the update (mixed V-cycle input)
@@ -145,6 +173,12 @@ struct PcgWork {
         r32_fresh = false;
         z32_fresh = false;
         // ptv may leave z in z32 (then with its dot fused); widen it wherever z itself is needed
+        auto take = [&](int slot) {  // the dot of slot, if left as partial sums
+            DeferredDot d = deferred;
+            deferred = DeferredDot{};
+            if (d.parts && d.slot != slot) fail(TG_ERR_STATE, id, "deferred dot in an unexpected slot");
+            return d;
+        };
         auto precond = [&](int slot) {
             const bool dotted = ptv(r.p, z.p, slot);
             if (z32_fresh && !dotted) {
@@ -159,12 +193,18 @@ struct PcgWork {
         int its = 0;
         double res = 0;
         for (its = 1; its <= maxIT; ++its) {
-            if (!atx(p.p, v.p, S_PV)) dot(p, v, S_PV);
+            defer_ok = true;
+            const bool pv_dotted = atx(p.p, v.p, S_PV);
+            defer_ok = false;
+            if (!pv_dotted) dot(p, v, S_PV);
+            const DeferredDot dpv = take(S_PV);
             if (r32) {
-                k_pcg_update<true><<<RED_BLOCKS, RED_THREADS, 0, st>>>(r, v, n, scal, sx1, S_PV, partial, r32);
+                k_pcg_update<true><<<RED_BLOCKS, RED_THREADS, 0, st>>>(r, v, n, scal, sx1, S_PV, partial, r32,
+                                                                       dpv.parts, dpv.n);
                 r32_fresh = true;
             } else {
-                k_pcg_update<false><<<RED_BLOCKS, RED_THREADS, 0, st>>>(r, v, n, scal, sx1, S_PV, partial, nullptr);
+                k_pcg_update<false><<<RED_BLOCKS, RED_THREADS, 0, st>>>(r, v, n, scal, sx1, S_PV, partial, nullptr,
+                                                                        dpv.parts, dpv.n);
             }
             k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR, d_hscal);
             TG_LAUNCH_CHECK(id);

@@ -82,6 +82,7 @@ struct Solver {
     void prolong_level(int c, const T *xc, T *xf, bool add);
     template <typename T>
     void vcycle_t(const T *r0, T *z0);
+    void finish_dot(const double *parts, int n, int slot);
     // r32_ready: rin32 already holds r0 rounded to FP32 (the CG update wrote it)
     bool vcycle(const double *r0, double *z0, int dot_slot = -1, bool r32_ready = false, bool z32_ok = false,
                 bool *z_in32 = nullptr);

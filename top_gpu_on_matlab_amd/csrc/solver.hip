@@ -532,6 +532,13 @@ void Solver::assemble() {
     assembled = true;
 }
 
+// A fused dot's partial sums: left for the PCG's next update to finish when it takes them, else summed
+// into pw.scal[slot] here.
+void Solver::finish_dot(const double *parts, int n, int slot) {
+    if (pw.defer_ok && !pw.deferred.parts) pw.deferred = DeferredDot{parts, n, slot};
+    else k_reduce_final<<<1, RED_THREADS, 0, st>>>(parts, n, pw.scal, slot);
+}
+
 // Grid of k_kelem_gather: one row per node plane (its in-plane and element indices are 32-bit).
 static dim3 gather_grid(const Grid &g) {
     if ((int64_t)g.NY * g.NX >= (1ll << 31) || g.elems() >= (1ll << 31) || g.NZ > 65535)
@@ -659,7 +666,7 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
             }
             if (ev) TG_HIP(hipEventRecord(ev[1], st), "tg_solver:kbyu");
             if (dp) {
-                k_reduce_final<<<1, RED_THREADS, 0, st>>>(dp, (int)gt, pw.scal, dot_slot);
+                finish_dot(dp, (int)gt, dot_slot);
                 dotted = true;
             }
         } else if (const_k0) {
@@ -797,7 +804,7 @@ void Solver::vcycle_t(const T *r0, T *z0) {
                 k_kfine_walsh<float, EPI_JACOBI, false, true, true><<<gt, 256, 0, st>>>(
                     x32, (float *)out, E32.p, D.nodeMask, kw32, (float)model.prm.cell_size, D.g, kt, r32, d32,
                     (float)omega, dotp.p, nullptr, r64_dot);
-            k_reduce_final<<<1, RED_THREADS, 0, st>>>(dotp, (int)gt, pw.scal, vc_dot_slot);
+            finish_dot(dotp, (int)gt, vc_dot_slot);
             vc_dotted = true;
             continue;
         }

@@ -27,6 +27,10 @@
 
 namespace tg {
 
+// "Not written yet" in the persistent solve's recurrence vectors: a signalling NaN, which arithmetic
+// never produces.
+constexpr unsigned long long COARSE_EMPTY = 0x7ff4c0a5e5e17e11ull;
+
 struct CoarseSolver {
     int axis = 0;                   // 0 = y, 1 = x, 2 = z
     int P = 0;                      // number of planes
@@ -40,6 +44,9 @@ struct CoarseSolver {
     int64_t ntot = 0;
     Grid g{};
     DevBuf<int32_t> lidx;           // per full-grid node: local node index within its plane (-1 none)
+    DevBuf<int32_t> gmap;           // per plane-order DOF: its full-grid DOF
+    DevBuf<int32_t> gzero;          // full-grid DOFs of inactive nodes (zero in every solution)
+    int64_t nzero = 0;
     DevBuf<int64_t> d_off, d_sOff, d_npad, d_nd;
     DevBuf<int32_t> rowPlane;       // plane of each DOF row (solve-time block products)
     DevBuf<double> Sinv;            // per plane: A_pp, then S_p, then S_p^-1 (row-major, padded)
@@ -69,12 +76,17 @@ struct CoarseSolver {
     void setup(const HostLevel &L, const Grid &grid, hipStream_t st);
     // Assemble the plane blocks from the level's element matrices (entry-major, ldk) and factorise.
     void factor(const double *Ks, int64_t ldk, const uint8_t *eleMask, const uint8_t *nodeMask, hipStream_t st);
-    // x = K_c^{-1} r on full-grid interleaved vectors (T = float or double; solve in FP64).
+    // x = K_c^{-1} r on full-grid interleaved vectors (T = float or double; solve in FP64).  packed: the
+    // plane-order right-hand side and the sentinel fill were written by the kernel that formed r (as
+    // plane_pack below describes), so the solve does not pack them again.
     template <typename T>
-    void solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st);
+    void solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st, bool packed = false);
+    // Whether the next solve runs as the persistent launch; if so, pk tells the kernel that forms the
+    // right-hand side where to put its plane-order copy.
+    bool plane_pack(PlanePack &pk, hipStream_t st);
     void enqueue_core(hipStream_t s) const;
     void build_persistent(hipStream_t st);
-    void enqueue_persistent(hipStream_t st);
+    void enqueue_persistent(hipStream_t st, double *x64, float *x32);
     // Fails (TG_ERR_HIP) if a persistent launch since the last check gave up waiting for an input (its
     // result is then invalid); synchronises the stream when there is something to check.
     void check_barrier(hipStream_t st);

@@ -473,8 +473,8 @@ void launch_chain(const ChainProb &A, const ChainProb &B, hipStream_t s) {
 // workgroups in proportion to their rows).  The forward stages fold the block-diagonal product into
 // the recurrence, z_p = S_p^-1 b_p - F_p z_{p-1}, and write zz; the separator and the backward stages,
 // x_p = z_p - B_p x_{p+1}, write xx.
-// Dataflow instead of barriers: before the launch (by the kernel that packs the right-hand side) zz and
-// xx are filled with a sentinel bit pattern (a signalling NaN that arithmetic never produces), every
+// Dataflow instead of barriers: before the launch (by the kernel that packs the right-hand side: the
+// V-cycle's restriction gather, else k_pack_planes) zz and xx are filled with a sentinel bit pattern (a signalling NaN that arithmetic never produces), every
 // entry is written once per launch, and a
 // workgroup that needs an earlier stage's vector polls exactly the entries it stages until they are no
 // longer the sentinel.  Stores and these loads are device-scope atomic accesses, coherent across the
@@ -494,6 +494,15 @@ struct PRow {
     const double *c;          // nullptr: none; else y[r] = c[r] + ... (written by an earlier stage)
     double *y;
     int n, n1, n2a, n2b;
+    int yoff;  // >= 0: the result is also the solution, full-grid DOF gmap[yoff + r] (else -1)
+};
+
+// The solution on the full grid (one of x64, x32), through the plane-order map; inactive DOFs zeroed.
+struct PersOut {
+    double *x64;
+    float *x32;
+    const int32_t *gmap, *gzero;
+    int nzero;
 };
 struct PStage {
     PRow q[2];
@@ -501,7 +510,7 @@ struct PStage {
     int gA;  // workgroups of q[0] when nq == 2
 };
 
-constexpr unsigned long long PERS_EMPTY = 0x7ff4c0a5e5e17e11ull;  // signalling NaN: "not written yet"
+constexpr unsigned long long PERS_EMPTY = COARSE_EMPTY;
 
 // The table's pointers are generic; these name them as global memory, so that the loads are global
 // (not flat) accesses.
@@ -563,10 +572,14 @@ __device__ __forceinline__ void stage_vec(double *dst, int n, const double *a, i
 
 template <int NK, bool NT>
 __global__ __launch_bounds__(256, 2) void k_coarse_persistent(const PStage *__restrict__ stages, int nstages, int L2,
-                                                              int *err, unsigned max_polls) {
+                                                              int *err, unsigned max_polls, const PersOut o) {
     extern __shared__ double sv[];  // two buffers of L2: [v2a; v2b] of stage s in buffer s & 1, zero past the ends
     constexpr int CH = 64 * NK;
     const int g = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = g * 256 + (int)threadIdx.x; i < o.nzero; i += (int)gridDim.x * 256) {  // inactive DOFs
+        if (o.x64) o.x64[o.gzero[i]] = 0.0;
+        else o.x32[o.gzero[i]] = 0.0f;
+    }
     auto chunk_len = [&](int64_t ld) { return (int)(((ld + CH - 1) / CH) * CH); };
     // NT: the factor rows are read once per solve; streaming (non-temporal) loads keep them from
     // displacing the finest level's data in the caches
@@ -581,7 +594,7 @@ __global__ __launch_bounds__(256, 2) void k_coarse_persistent(const PStage *__re
     // workgroup has rows at all, and the stage's operands that depend on no earlier stage, in
     // registers: the M1 row with this lane's entries of v1 (ld1 <= CH: one chunk) and the first M2 chunk.
     struct Cur {
-        int qi, r;
+        int qi, r, gi;  // gi: lane 0's full-grid DOF of the row's solution entry (yoff >= 0)
         bool any;
         double m1[NK], v1[NK], m2[NK];
     };
@@ -592,6 +605,7 @@ __global__ __launch_bounds__(256, 2) void k_coarse_persistent(const PStage *__re
         c.r = 4 * (c.qi ? g - S.gA : g) + w;
         const PRow &q = S.q[c.qi];
         c.any = c.r - w < q.n;
+        c.gi = (q.yoff >= 0 && lane == 0 && c.r < q.n) ? o.gmap[q.yoff + c.r] : 0;
         if (c.r < q.n) {
             if (q.M1) {
                 load_chunk(q.M1 + (int64_t)c.r * q.ld1, q.ld1, 0, c.m1);
@@ -630,8 +644,15 @@ __global__ __launch_bounds__(256, 2) void k_coarse_persistent(const PStage *__re
                 }
             }
             double t = t1 - t2;
-            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
-            if (lane == 0) coh_store(q.y + r, cval + t);
+            for (int sh = 32; sh > 0; sh >>= 1) t += __shfl_xor(t, sh, 64);
+            if (lane == 0) {
+                const double val = cval + t;
+                coh_store(q.y + r, val);
+                if (q.yoff >= 0) {
+                    if (o.x64) o.x64[c.gi] = val;
+                    else o.x32[c.gi] = (float)val;
+                }
+            }
         }
     };
     // Two stages in flight: part A of stage s + 1 (its loads) is issued before part B of stage s waits
@@ -747,6 +768,26 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
     std::vector<int32_t> rp(std::max<int64_t>(ntot, 1), 0);
     for (int p = 0; p < P; ++p)
         for (int64_t r = off[p]; r < off[p + 1]; ++r) rp[r] = p;
+    // plane-order DOF -> full-grid DOF, and the inactive nodes' DOFs (32-bit: the persistent solve's
+    // output map; it is used only where 3 x the nodes fit)
+    if (3 * g.nodes() < (1ll << 31)) {
+        std::vector<int32_t> gm(std::max<int64_t>(ntot, 1), 0), gz;
+        for (int64_t n = 0; n < g.nodes(); ++n) {
+            const int64_t y = n % g.NY, x = (n / g.NY) % g.NX, z = n / ((int64_t)g.NY * g.NX);
+            const int p = (int)(axis == 0 ? y : (axis == 1 ? x : z));
+            for (int c = 0; c < 3; ++c) {
+                if (lid[n] >= 0) gm[off[p] + 3 * (int64_t)lid[n] + c] = (int32_t)(3 * n + c);
+                else gz.push_back((int32_t)(3 * n + c));
+            }
+        }
+        gmap.upload(gm, st);
+        nzero = (int64_t)gz.size();
+        if (nzero) gzero.upload(gz, st);
+    } else {
+        gmap.release();
+        gzero.release();
+        nzero = 0;
+    }
     lidx.upload(lid, st);
     d_off.upload(off, st);
     d_sOff.upload(sOff, st);
@@ -936,6 +977,7 @@ void CoarseSolver::build_persistent(hipStream_t st) {
     for (int p = 0; p < P; ++p) maxpad = std::max(maxpad, npad[p]);
     const int nk = (int)((maxpad + 63) / 64);
     if (nk < 1 || nk > 16) return;  // planes wider than 1024 DOFs: step by step
+    if (!gmap.p) return;            // no 32-bit output map (more than 2^31 / 3 nodes)
     PRow none{};
     // forward results (and the chains' inputs) in zz = yy, the separator's and backward results in xx
     const double *zz = yy.p;
@@ -945,6 +987,7 @@ void CoarseSolver::build_persistent(hipStream_t st) {
         PRow q{};
         q.n = (int)nd[p];
         q.y = y + off[p];
+        q.yoff = y == xxp ? (int)off[p] : -1;  // the separator's and backward results are the solution
         if (diag) {
             q.M1 = Sinv.p + sOff[p];
             q.ld1 = npad[p];
@@ -1041,19 +1084,20 @@ void CoarseSolver::build_persistent(hipStream_t st) {
     pers_ok = true;
 }
 
-void CoarseSolver::enqueue_persistent(hipStream_t st) {
+void CoarseSolver::enqueue_persistent(hipStream_t st, double *x64, float *x32) {
     const char *id = "tg_solver:coarse";
     const size_t lds = 2 * (size_t)pers_L2 * sizeof(double);
     const PStage *tab = (const PStage *)pers_table.p;
+    const PersOut o{x64, x32, gmap.p, gzero.p, (int)nzero};
     switch (pers_nk) {
 #define TG_PERS_CASE(K)                                                                                   \
     case K:                                                                                               \
         if (pers_streaming)                                                                                \
             k_coarse_persistent<K, true><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L2, pers_err.p, \
-                                                                        pers_polls);                        \
+                                                                        pers_polls, o);                     \
         else                                                                                              \
             k_coarse_persistent<K, false><<<pers_groups, 256, lds, st>>>(tab, pers_stages, pers_L2, pers_err.p, \
-                                                                         pers_polls);                       \
+                                                                         pers_polls, o);                    \
         break;
         TG_PERS_CASE(1) TG_PERS_CASE(2) TG_PERS_CASE(3) TG_PERS_CASE(4) TG_PERS_CASE(5) TG_PERS_CASE(6)
         TG_PERS_CASE(7) TG_PERS_CASE(8) TG_PERS_CASE(9) TG_PERS_CASE(10) TG_PERS_CASE(11) TG_PERS_CASE(12)
@@ -1079,17 +1123,31 @@ void CoarseSolver::check_barrier(hipStream_t st) {
     }
 }
 
+bool CoarseSolver::plane_pack(PlanePack &pk, hipStream_t st) {
+    if (!factored || !use_persistent || use_graph) return false;
+    if (!pers_built) build_persistent(st);
+    if (!pers_ok) return false;
+    pk = PlanePack{bb.p, yy.p, xx.p, lidx.p, d_off.p, axis, PERS_EMPTY};
+    return true;
+}
+
 template <typename T>
-void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st) {
+void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t st, bool packed) {
     const char *id = "tg_solver:coarse";
     if (!factored) fail(TG_ERR_STATE, id, "coarsest level not factorised (call assemble first)");
     if (use_persistent && !use_graph && !pers_built) build_persistent(st);
     const bool pers = use_persistent && !use_graph && pers_ok;
-    k_pack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(r, bb, nodeMask, lidx, d_off, g, axis,
-                                                               pers ? yy.p : nullptr, pers ? xx.p : nullptr);
-    TG_LAUNCH_CHECK(id);
+    if (packed && !pers) fail(TG_ERR_STATE, id, "right-hand side packed for a persistent solve that is not set up");
+    if (!packed) {
+        k_pack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(r, bb, nodeMask, lidx, d_off, g, axis,
+                                                                   pers ? yy.p : nullptr, pers ? xx.p : nullptr);
+        TG_LAUNCH_CHECK(id);
+    }
     if (pers) {
-        enqueue_persistent(st);
+        // the launch writes the solution on the full grid itself (no unpack)
+        if constexpr (sizeof(T) == 8) enqueue_persistent(st, (double *)x, nullptr);
+        else enqueue_persistent(st, nullptr, (float *)x);
+        return;
     } else if (!use_graph) {
         enqueue_core(st);
     } else {
@@ -1107,11 +1165,11 @@ void CoarseSolver::solve(const T *r, T *x, const uint8_t *nodeMask, hipStream_t 
         }
         TG_HIP(hipGraphLaunch(exec, st), id);
     }
-    k_unpack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(pers ? xx : yy, x, nodeMask, lidx, d_off, g, axis);
+    k_unpack_planes<T><<<grid_for(g.nodes(), 256), 256, 0, st>>>(yy, x, nodeMask, lidx, d_off, g, axis);
     TG_LAUNCH_CHECK(id);
 }
 
-template void CoarseSolver::solve<double>(const double *, double *, const uint8_t *, hipStream_t);
-template void CoarseSolver::solve<float>(const float *, float *, const uint8_t *, hipStream_t);
+template void CoarseSolver::solve<double>(const double *, double *, const uint8_t *, hipStream_t, bool);
+template void CoarseSolver::solve<float>(const float *, float *, const uint8_t *, hipStream_t, bool);
 
 }  // namespace tg

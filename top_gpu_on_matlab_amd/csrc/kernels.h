@@ -253,11 +253,23 @@ __global__ void k_pack_sym(const S *__restrict__ K, T *__restrict__ Kp, int64_t 
     Kp[idx] = (T)K[(int64_t)(i + 24 * j) * ldk + e];
 }
 
+// The coarsest level's right-hand side in plane order, written by the gather that forms it (when set):
+// active node n of plane p -> b[off[p] + 3 lidx[n] + c] in FP64, and the persistent solve's recurrence
+// vectors e1, e2 set to the "not written" sentinel at the same slots (every slot is an active node's).
+struct PlanePack {
+    double *b = nullptr, *e1 = nullptr, *e2 = nullptr;
+    const int32_t *lidx = nullptr;
+    const int64_t *off = nullptr;
+    int axis = 0;
+    unsigned long long empty = 0;
+};
+
 template <typename T, int EPI, bool MASKED>
 __global__ __launch_bounds__(256) void k_kelem_gather(const T *__restrict__ f, int64_t ldk, T *__restrict__ out,
                                                       const T *__restrict__ u, const uint8_t *__restrict__ eleMask,
                                                       const uint8_t *__restrict__ mask, const Grid g,
-                                                      const T *__restrict__ r, const T *__restrict__ diag, T omega) {
+                                                      const T *__restrict__ r, const T *__restrict__ diag, T omega,
+                                                      const PlanePack pk = PlanePack{}) {
     // one thread per node of node plane blockIdx.y; in-plane and element indices in 32 bits (the host
     // checks that they fit), neighbour elements by added strides
     const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
@@ -299,6 +311,17 @@ __global__ __launch_bounds__(256) void k_kelem_gather(const T *__restrict__ f, i
         if (mk & 4) acc[2] = u[3 * n + 2];
     }
     epilogue<T, EPI>(out, n, acc, r, diag, u, omega);
+    if (EPI == EPI_KU && pk.b) {
+        const int p = pk.axis == 0 ? iy : (pk.axis == 1 ? ix : iz);
+        const int64_t base = pk.off[p] + 3 * (int64_t)pk.lidx[n];
+        const double empty = __longlong_as_double((long long)pk.empty);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            pk.b[base + c] = (double)acc[c];  // the value stored above (EPI_KU)
+            pk.e1[base + c] = empty;
+            pk.e2[base + c] = empty;
+        }
+    }
 }
 
 // Trilinear weight of coarse vertex j at fine offset (a, b, c) of a span-S block; exact in binary.

@@ -77,7 +77,8 @@ struct Solver {
                     T *xout = nullptr);
     bool fine_tiled() const { return !super && kfine_kernel == 0 && walsh_ok; }
     template <typename T>
-    void restrict_level(int c, const T *rf, T *rc);
+    // pk (optional): also write the coarsest level's plane-order copy for its persistent solve
+    void restrict_level(int c, const T *rf, T *rc, const PlanePack *pk = nullptr);
     template <typename T>
     void prolong_level(int c, const T *xc, T *xf, bool add);
     template <typename T>

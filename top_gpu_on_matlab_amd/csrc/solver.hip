@@ -705,7 +705,7 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
 }
 
 template <typename T>
-void Solver::restrict_level(int c, const T *rf, T *rc) {
+void Solver::restrict_level(int c, const T *rf, T *rc, const PlanePack *pk) {
     DevLevel &C = lv[c], &F = lv[c - 1];
     const T *cf = sizeof(T) == 8 ? (const T *)C.coeffF64.p : (const T *)C.coeffF32.p;
     T *fv = sizeof(T) == 8 ? (T *)C.fv64.p : (T *)C.fv32.p;
@@ -714,7 +714,7 @@ void Solver::restrict_level(int c, const T *rf, T *rc) {
     if (C.span == 4) k_restrict_tiled<T, 4><<<gt, RT_THREADS, 0, st>>>(rf, cf, C.g, F.g, fv, C.ldk);
     else k_restrict_tiled<T, 2><<<gt, RT_THREADS, 0, st>>>(rf, cf, C.g, F.g, fv, C.ldk);
     k_kelem_gather<T, EPI_KU, false><<<gather_grid(C.g), 256, 0, st>>>(
-        fv, C.ldk, rc, nullptr, C.eleMask, C.nodeMask, C.g, nullptr, nullptr, (T)0);
+        fv, C.ldk, rc, nullptr, C.eleMask, C.nodeMask, C.g, nullptr, nullptr, (T)0, pk ? *pk : PlanePack{});
     TG_LAUNCH_CHECK("tg_solver:restrict");
 }
 
@@ -752,26 +752,30 @@ void Solver::vcycle_t(const T *r0, T *z0) {
     rr[0] = r0;
     for (int i = 1; i < L; ++i) rr[i] = vec<T>(lv[i], 0);
     for (int i = 0; i < L; ++i) xx[i] = vec<T>(lv[i], 1);
+    // the restriction into the coarsest level also packs its persistent solve's right-hand side
+    PlanePack pk;
+    const bool packed = coarse.plane_pack(pk, st);
     for (int i = 0; i < L - 1; ++i) {
+        const PlanePack *pki = (packed && i + 1 == L - 1) ? &pk : nullptr;
         DevLevel &D = lv[i];
         const int64_t nv = 3 * D.g.nodes();
         if (i == 0 && !adapted && fine_tiled()) {
             // pre-smoothing x = omega r ./ diag fused into the residual product
             T *d = vec<T>(D, 2);
             kbyu_level<T>(0, nullptr, d, EPI_RESID_JAC0, rr[0], diag_of<T>(D), -1, xx[0]);
-            restrict_level<T>(1, d, (T *)rr[1]);
+            restrict_level<T>(1, d, (T *)rr[1], pki);
             continue;
         }
         k_jacobi0<T><<<grid_for(nv, 256), 256, 0, st>>>(rr[i], diag_of<T>(D), xx[i], w, nv);
         if (adapted) {
-            restrict_level<T>(i + 1, rr[i], (T *)rr[i + 1]);
+            restrict_level<T>(i + 1, rr[i], (T *)rr[i + 1], pki);
         } else {
             T *d = vec<T>(D, 2);
             kbyu_level<T>(i, xx[i], d, EPI_RESID, rr[i], nullptr);
-            restrict_level<T>(i + 1, d, (T *)rr[i + 1]);
+            restrict_level<T>(i + 1, d, (T *)rr[i + 1], pki);
         }
     }
-    coarse.solve<T>(rr[L - 1], xx[L - 1], lv[L - 1].nodeMask, st);
+    coarse.solve<T>(rr[L - 1], xx[L - 1], lv[L - 1].nodeMask, st, packed);
     for (int i = L - 2; i >= 0; --i) {
         DevLevel &D = lv[i];
         prolong_level<T>(i + 1, xx[i + 1], xx[i], true);

@@ -625,16 +625,20 @@ __global__ __launch_bounds__(RED_THREADS) void k_dot_partial(const T *__restrict
 
 // out[slot] = sum(partial[0..nb))
 // host (optional): a device-mapped pinned host copy of out, written too, so the host reads the sum
-// without a copy
+// without a copy; seq_ptr (optional): then seq is published there (a system-scope release store, so a
+// host that sees seq also sees the sum)
 static __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__restrict__ partial, int nb,
                                                               double *__restrict__ out, int slot,
-                                                              double *__restrict__ host = nullptr) {
+                                                              double *__restrict__ host = nullptr,
+                                                              unsigned long long *seq_ptr = nullptr,
+                                                              unsigned long long seq = 0) {
     double s = 0;
     for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[i];
     s = block_sum<double>(s);
     if (threadIdx.x == 0) {
         out[slot] = s;
         if (host) host[slot] = s;
+        if (seq_ptr) __hip_atomic_store(seq_ptr, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 

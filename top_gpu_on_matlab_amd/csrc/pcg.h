@@ -118,6 +118,7 @@ struct PcgWork {
     DeferredDot deferred;       // ... here, for the update to finish
     double *h_scal = nullptr;  // pinned host copies of the scalars the host reads (S_BB, S_RR) ...
     double *d_hscal = nullptr;  // ... and their device address, written by the final reductions
+    unsigned long long seq = 0;  // the last publish number of S_RR (at h_scal[16], device d_hscal[16])
     float *r32 = nullptr;     // optional FP32 copy of r kept by the update (mixed V-cycle input)
     bool r32_fresh = false;   // r32 holds the current r
     float *z32 = nullptr;     // optional FP32 preconditioner output (mixed V-cycle) ...
@@ -134,8 +135,9 @@ struct PcgWork {
         scal.alloc(16);
         scal.zero(st);
         if (!h_scal) {
-            TG_HIP(hipHostMalloc((void **)&h_scal, 16 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent),
+            TG_HIP(hipHostMalloc((void **)&h_scal, 17 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent),
                    "tg_pcg:alloc");
+            h_scal[16] = 0.0;  // publish number 0
             TG_HIP(hipHostGetDevicePointer((void **)&d_hscal, h_scal, 0), "tg_pcg:alloc");
         }
     }
@@ -152,6 +154,28 @@ struct PcgWork {
     double read(int slot) {
         TG_HIP(hipStreamSynchronize(st), "tg_pcg");
         return h_scal[slot];
+    }
+    // r.r of the update: reduced with a publish number, which the host polls in pinned memory (no
+    // stream synchronisation in the loop).  The stream is queried now and then, so a device error, or
+    // a stream that finished without publishing, ends the wait.
+    void reduce_rr() {
+        ++seq;
+        k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR, d_hscal,
+                                                  (unsigned long long *)(d_hscal + 16), seq);
+    }
+    double wait_rr() {
+        unsigned long long *hs = (unsigned long long *)(h_scal + 16);
+        for (unsigned spins = 1; __atomic_load_n(hs, __ATOMIC_ACQUIRE) != seq; ++spins) {
+            if ((spins & 4095) == 0) {
+                const hipError_t e = hipStreamQuery(st);
+                if (e != hipErrorNotReady) {
+                    TG_HIP(e, "tg_pcg");
+                    if (__atomic_load_n(hs, __ATOMIC_ACQUIRE) != seq)
+                        fail(TG_ERR_HIP, "tg_pcg", "residual norm not published by a finished stream");
+                }
+            }
+        }
+        return h_scal[S_RR];
     }
 
     // atx(u, out, slot): out = A u;  ptv(r, z, slot): z = M r.  Each returns true when it has also
@@ -206,9 +230,9 @@ struct PcgWork {
                 k_pcg_update<false><<<RED_BLOCKS, RED_THREADS, 0, st>>>(r, v, n, scal, sx1, S_PV, partial, nullptr,
                                                                         dpv.parts, dpv.n);
             }
-            k_reduce_final<<<1, RED_THREADS, 0, st>>>(partial, RED_BLOCKS, scal, S_RR, d_hscal);
+            reduce_rr();
             TG_LAUNCH_CHECK(id);
-            res = std::sqrt(read(S_RR)) / normB;
+            res = std::sqrt(wait_rr()) / normB;
             if (!std::isfinite(res)) fail(TG_ERR_NUMERIC, id, "non-finite residual at iteration %d", its);
             if (res < tol) {
                 k_pcg_axpy<<<grid_for(n, 256), 256, 0, st>>>(y, p, n, scal, sx1, S_PV);

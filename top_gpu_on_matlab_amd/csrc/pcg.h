@@ -137,7 +137,7 @@ struct PcgWork {
         if (!h_scal) {
             TG_HIP(hipHostMalloc((void **)&h_scal, 17 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent),
                    "tg_pcg:alloc");
-            h_scal[16] = 0.0;  // publish number 0
+            *(unsigned long long *)(h_scal + 16) = 0ull;  // publish number 0
             TG_HIP(hipHostGetDevicePointer((void **)&d_hscal, h_scal, 0), "tg_pcg:alloc");
         }
     }
@@ -196,6 +196,8 @@ struct PcgWork {
         int sx1 = S_XA, sx2 = S_XB;
         r32_fresh = false;
         z32_fresh = false;
+        defer_ok = false;
+        deferred = DeferredDot{};
         // ptv may leave z in z32 (then with its dot fused); widen it wherever z itself is needed
         auto take = [&](int slot) {  // the dot of slot, if left as partial sums
             DeferredDot d = deferred;
@@ -217,9 +219,15 @@ struct PcgWork {
         int its = 0;
         double res = 0;
         for (its = 1; its <= maxIT; ++its) {
-            defer_ok = true;
-            const bool pv_dotted = atx(p.p, v.p, S_PV);
-            defer_ok = false;
+            bool pv_dotted;
+            {
+                struct Defer {  // deferral allowed for this operator call only (also when it throws)
+                    bool &f;
+                    explicit Defer(bool &x) : f(x) { f = true; }
+                    ~Defer() { f = false; }
+                } allow(defer_ok);
+                pv_dotted = atx(p.p, v.p, S_PV);
+            }
             if (!pv_dotted) dot(p, v, S_PV);
             const DeferredDot dpv = take(S_PV);
             if (r32) {

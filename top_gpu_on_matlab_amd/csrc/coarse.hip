@@ -770,8 +770,9 @@ void CoarseSolver::setup(const HostLevel &L, const Grid &grid, hipStream_t st) {
         for (int64_t r = off[p]; r < off[p + 1]; ++r) rp[r] = p;
     // plane-order DOF -> full-grid DOF, and the inactive nodes' DOFs (32-bit: the persistent solve's
     // output map; it is used only where 3 x the nodes fit)
+    std::vector<int32_t> gm, gz;  // kept until the stream synchronisation below (asynchronous uploads)
     if (3 * g.nodes() < (1ll << 31)) {
-        std::vector<int32_t> gm(std::max<int64_t>(ntot, 1), 0), gz;
+        gm.assign(std::max<int64_t>(ntot, 1), 0);
         for (int64_t n = 0; n < g.nodes(); ++n) {
             const int64_t y = n % g.NY, x = (n / g.NY) % g.NX, z = n / ((int64_t)g.NY * g.NX);
             const int p = (int)(axis == 0 ? y : (axis == 1 ? x : z));

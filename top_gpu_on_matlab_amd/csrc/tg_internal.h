@@ -89,9 +89,14 @@ struct DevBuf {
         p = nullptr;
         n = 0;
     }
+    // The copy has completed when this returns, so the source may be a temporary (uploads are set-up
+    // work, outside the solver loops).
     void upload(const T *h, size_t count, hipStream_t s) {
         if (count > n) alloc(count);
-        if (count) TG_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "topgpu:upload");
+        if (count) {
+            TG_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s), "topgpu:upload");
+            TG_HIP(hipStreamSynchronize(s), "topgpu:upload");
+        }
     }
     void upload(const std::vector<T> &h, hipStream_t s) { upload(h.data(), h.size(), s); }
     void zero(hipStream_t s) {

@@ -177,11 +177,12 @@ def test_coarse_solve_graph_replay_matches_direct_launches(pair):
 
 
 def test_coarse_solve_persistent_matches_step_by_step(pair):
-    """The coarsest solve as one persistent launch (stages meeting at a grid barrier, the block-diagonal
-    product folded into the forward recurrence) against the step-by-step launches on the same factors:
-    the same recurrences summed in another order, so equal to rounding (1e-12 relative); and repeated
-    persistent launches, whose barrier counter carries over from launch to launch, give bitwise the
-    same result every time, also after step-by-step launches in between."""
+    """The coarsest solve as one persistent launch (stages passing the recurrence vectors to each other,
+    the block-diagonal product folded into the forward recurrence) against the step-by-step launches on
+    the same factors: the same recurrences summed in another order, so equal to rounding (1e-12
+    relative); and repeated persistent launches, whose vectors are refilled with the "not written"
+    sentinel for every solve, give bitwise the same result every time, also after step-by-step
+    launches in between."""
     name, om, osv, gm, gsv, E = pair
     r = np.random.default_rng(5).standard_normal(3 * om.levels[-1].numNodes)
     try:
@@ -207,6 +208,27 @@ def test_vcycle(pair):
     name, om, osv, gm, gsv, E = pair
     r = np.random.default_rng(4).standard_normal(3 * om.levels[0].numNodes)
     assert _rel(gsv.vcycle(r).cpu().numpy(), osv.vcycle(r)) < 1e-9
+
+
+def test_vcycle_coarse_solve_forms_agree(pair):
+    """The V-cycle with its coarsest solve as the persistent launch (the right-hand side packed into
+    plane order by the restriction's last gather, the solution written to the full grid by the
+    launch itself) and step by step (separate pack and unpack launches): equal to rounding, and
+    switching back to the persistent form reproduces its result bitwise."""
+    name, om, osv, gm, gsv, E = pair
+    r = np.random.default_rng(6).standard_normal(3 * om.levels[0].numNodes)
+    try:
+        pers = gsv.vcycle(r).clone()
+        gsv.set_tuning(coarse_persistent=False)
+        step = gsv.vcycle(r).clone()
+        gsv.set_tuning(coarse_persistent=True)
+        again = gsv.vcycle(r).clone()
+    finally:
+        gsv.set_tuning(coarse_persistent=True)
+    print('[%s] V-cycle, persistent vs step-by-step coarsest solve: %.2e'
+          % (name, _rel(pers.cpu().numpy(), step.cpu().numpy())), flush=True)
+    assert _rel(pers.cpu().numpy(), step.cpu().numpy()) < 1e-10, name
+    assert pers.equal(again), name
 
 
 def test_pcg_parity_and_determinism(pair):

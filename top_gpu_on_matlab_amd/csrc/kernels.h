@@ -334,60 +334,6 @@ __device__ inline T ri_w(int a, int b, int c, int j) {
     return wy * wx * wz;
 }
 
-// Restriction r_c = R r_f (Solving_RestrictResidual, TOP_GPU.m:817-847): for every coarse node J,
-// sum over the non-empty coarse elements around J of RI(:,J)' * (r_f ./ coeff) on the element's
-// (S+1)^3 fine nodes.  coeff holds transferMatCoeffi per fine grid node (1 where unused).
-// Eight lanes per coarse node, one per adjacent coarse element (lane k: element offset
-// (k&1, k>>1&1, k>>2) - 1); each walks the S^3 fine nodes where J's trilinear weight is nonzero,
-// and the eight partial sums are combined in a fixed tree order.
-template <typename T, int S>
-__global__ __launch_bounds__(256) void k_restrict(const T *__restrict__ rf, T *__restrict__ rc,
-                                                  const T *__restrict__ coeff, const uint8_t *__restrict__ cMask,
-                                                  const uint8_t *__restrict__ cEle, const Grid gc, const Grid gf) {
-    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t J = gid >> 3;
-    const int sub = (int)(gid & 7);
-    const bool valid = J < gc.nodes();
-    T a0 = 0, a1 = 0, a2 = 0;
-    if (valid && (cMask[J] & NODE_ACTIVE)) {
-        const int jy = (int)(J % gc.NY);
-        const int64_t t = J / gc.NY;
-        const int jx = (int)(t % gc.NX), jz = (int)(t / gc.NX);
-        const int oy = (sub & 1) - 1, ox = ((sub >> 1) & 1) - 1, oz = (sub >> 2) - 1;
-        const int ey = jy + oy, ex = jx + ox, ez = jz + oz;
-        if ((unsigned)ey < (unsigned)gc.EY && (unsigned)ex < (unsigned)gc.EX && (unsigned)ez < (unsigned)gc.EZ &&
-            cEle[ey + (int64_t)gc.EY * (ex + (int64_t)gc.EX * ez)]) {
-            const int jv = vtx_of(-oy, -ox, -oz);
-            const int vy = vtx_y(jv), vx = vtx_x(jv), vz = vtx_z(jv);  // nonzero weights: offsets v..v+S-1
-            const int64_t base = (int64_t)(S * ey + vy) + (int64_t)gf.NY * ((S * ex + vx) + (int64_t)gf.NX * (S * ez + vz));
-#pragma unroll
-            for (int c = 0; c < S; ++c)
-#pragma unroll
-                for (int b = 0; b < S; ++b)
-#pragma unroll
-                    for (int a = 0; a < S; ++a) {
-                        const T w = ri_w<T, S>(a + vy, b + vx, c + vz, jv);
-                        const int64_t i = base + a + (int64_t)gf.NY * (b + (int64_t)gf.NX * c);
-                        const T cf = coeff[i];
-                        a0 += w * (rf[3 * i] / cf);
-                        a1 += w * (rf[3 * i + 1] / cf);
-                        a2 += w * (rf[3 * i + 2] / cf);
-                    }
-        }
-    }
-#pragma unroll
-    for (int o = 1; o < 8; o <<= 1) {
-        a0 += __shfl_down(a0, o, 8);
-        a1 += __shfl_down(a1, o, 8);
-        a2 += __shfl_down(a2, o, 8);
-    }
-    if (valid && sub == 0) {
-        rc[3 * J] = a0;
-        rc[3 * J + 1] = a1;
-        rc[3 * J + 2] = a2;
-    }
-}
-
 // Workgroups go to the 8 XCDs of the MI355X round-robin by blockIdx.x (workgroup 8 k + x is the k-th
 // one XCD x runs).  This renumbering gives XCD x a contiguous range of [0, n), so neighbouring tiles,
 // which share halo data, meet in the same L2.  A bijection of [0, n) for any n.
@@ -397,7 +343,11 @@ __device__ __forceinline__ int xcd_contiguous(int b, int n) {
     return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
 }
 
-// Restriction, element-centric and LDS-tiled (same sums as k_restrict, in separable order).  A
+// Restriction r_c = R r_f (Solving_RestrictResidual, TOP_GPU.m:817-847): for every coarse node J, the
+// sum over the non-empty coarse elements around J of RI(:,J)' * (r_f ./ coeff) on the element's
+// (S+1)^3 fine nodes; coeff is transferMatCoeffi per fine grid node, a count of at most 8 non-empty
+// coarse elements (1 where unused), stored as one byte and converted exactly to T.
+// Element-centric and LDS-tiled, the trilinear weights applied in separable order.  A
 // workgroup owns RT_E x RT_E coarse elements (y, x) over RT_Z coarse layers and marches through the
 // fine node planes of that box: each plane of v = r_f ./ coeff is staged in LDS (the next one is
 // prefetched into registers), every lane forms, for its (element, vertex column (jy, jx), component),
@@ -408,7 +358,7 @@ __device__ __forceinline__ int xcd_contiguous(int b, int n) {
 constexpr int RT_E = 8, RT_Z = 2;
 constexpr int RT_THREADS = RT_E * RT_E * 12;  // one lane per (element, vertex column, component)
 template <typename T, int S>
-__global__ __launch_bounds__(RT_THREADS) void k_restrict_tiled(const T *__restrict__ rf, const T *__restrict__ coeff,
+__global__ __launch_bounds__(RT_THREADS) void k_restrict_tiled(const T *__restrict__ rf, const uint8_t *__restrict__ coeff,
                                                                const Grid gc, const Grid gf, T *__restrict__ f,
                                                                int64_t ldk) {
     constexpr int FN = RT_E * S + 1;                      // fine nodes per tile side
@@ -430,7 +380,8 @@ __global__ __launch_bounds__(RT_THREADS) void k_restrict_tiled(const T *__restri
     const bool own = ey < gc.EY && ex < gc.EX;
     // the next plane's raw values (the division waits for them, so it is done at the LDS store, a
     // plane later, and the loads stay in flight meanwhile); out-of-grid slots read node 0 and store 0
-    T preR[NI], preC[NI];
+    T preR[NI];
+    uint8_t preC[NI];
     bool preOk[NI];
     auto fetch = [&](int iz) {
 #pragma unroll
@@ -454,7 +405,7 @@ __global__ __launch_bounds__(RT_THREADS) void k_restrict_tiled(const T *__restri
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
             const int i = t + RT_THREADS * k;
-            if (i < NV) sV[(i % 3) * FN * FN + i / 3] = preOk[k] ? preR[k] / preC[k] : (T)0;
+            if (i < NV) sV[(i % 3) * FN * FN + i / 3] = preOk[k] ? preR[k] / (T)preC[k] : (T)0;
         }
         lds_barrier();
         if (iz < iz1) fetch(iz + 1);

@@ -22,8 +22,8 @@ struct DevLevel {
     DevBuf<float> Ks32;                    // mixed mode: the same in FP32
     DevBuf<double> fv64;                   // per-element K*u products, entry-major [24 x ldk]
     DevBuf<float> fv32;
-    DevBuf<double> coeffF64;               // transferMatCoeffi on the finer grid's nodes (levels >= 2)
-    DevBuf<float> coeffF32;
+    DevBuf<uint8_t> coeffCnt;              // transferMatCoeffi on the finer grid's nodes (levels >= 2): the
+                                           // number of non-empty coarse elements around the node, 1 where unused
     DevBuf<double> P;                      // Pch / Psub [24 x 24 x span^3] (levels >= 2)
 };
 

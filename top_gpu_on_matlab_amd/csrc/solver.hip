@@ -304,16 +304,18 @@ Solver::Solver(const tg_model &m, hipStream_t s) : model(m), st(s) {
             } else if (i < L - 1) {
                 D.Ks64p.alloc(KSYM_N * D.ldk);  // packed symmetric (FP64 V-cycle; the coarsest level has no K*u)
             }
-            // transfer coefficients on the finer grid (1 where a node is unused)
+            // transfer coefficients on the finer grid (1 where a node is unused): counts of at most 8
+            // coarse elements, so one byte each, converted exactly to T where the restriction divides
             const HostLevel &F = m.lv[i - 1];
-            std::vector<double> cf(F.nodesFull(), 1.0);
+            std::vector<uint8_t> cf(F.nodesFull(), 1);
             for (int64_t n = 0; n < F.nodesFull(); ++n)
-                if (H.fineInterId[n]) cf[n] = H.coeff[H.fineInterId[n] - 1];
-            D.coeffF64.upload(cf, st);
-            if (prec == TG_PREC_MIXED) {
-                std::vector<float> c32(cf.begin(), cf.end());
-                D.coeffF32.upload(c32, st);
-            }
+                if (H.fineInterId[n]) {
+                    const double c = H.coeff[H.fineInterId[n] - 1];
+                    if (!(c >= 1 && c <= 8 && c == (double)(int)c))
+                        fail(TG_ERR_STATE, id, "transfer coefficient %g out of range", c);
+                    cf[n] = (uint8_t)c;
+                }
+            D.coeffCnt.upload(cf, st);
             std::vector<double> P;
             build_pch(H.span, P);
             D.P.upload(P, st);
@@ -707,7 +709,7 @@ bool Solver::kbyu_level(int i, const T *u, T *out, int epi, const T *rr, const T
 template <typename T>
 void Solver::restrict_level(int c, const T *rf, T *rc, const PlanePack *pk) {
     DevLevel &C = lv[c], &F = lv[c - 1];
-    const T *cf = sizeof(T) == 8 ? (const T *)C.coeffF64.p : (const T *)C.coeffF32.p;
+    const uint8_t *cf = C.coeffCnt.p;
     T *fv = sizeof(T) == 8 ? (T *)C.fv64.p : (T *)C.fv32.p;
     const unsigned gt = (unsigned)(((C.g.EY + RT_E - 1) / RT_E) * ((C.g.EX + RT_E - 1) / RT_E) *
                                    ((C.g.EZ + RT_Z - 1) / RT_Z));

@@ -222,6 +222,45 @@ def test_coarse_solve_and_vcycle_symmetry(setup):
     assert abs(lhs - rhs) < 1e-9 * abs(lhs)
 
 
+@pytest.mark.parametrize('vtype', ['Standard', 'Adapted'])
+def test_vcycle_equals_its_matrix_form(vtype):
+    """Solving_Vcycle (TOP_GPU.m:783-815) as matrices, on a 4 x 8 x 4 cantilever with three levels.  With
+    S_l = omega D_l^-1, the Standard cycle is M_l = S_l + (I - S_l A_l)(S_l + P_l M_(l+1) R_l (I - A_l S_l)),
+    the Adapted one M_l = 2 S_l + P_l M_(l+1) R_l, and M_L = A_L^-1.  A_1 is assembled directly on the
+    voxel grid; A_2 is the oracle's level-2 operator, D_l the diagonals (each pinned above), R_l and P_l
+    the oracle's transfers applied to unit vectors, A_L^-1 the inverse of the coarsest matrix.  The
+    oracle's level loop must agree to 1e-12: a dropped smoothing term, a wrong weight, or a residual
+    taken from the wrong level does not."""
+    m = M.build_cuboid_model(4, 8, 4, opt_case=1)
+    assert m.numLevels == 3
+    E = random_modulus(m.levels[0].numElements, seed=4)
+    sv = S.Solver(m, vcycle_type=vtype)
+    sv.assemble(E)
+    n = [3 * lv.numNodes for lv in m.levels]
+    w = sv.omega
+    A1 = direct_global_K(m, E).toarray()
+    A2 = np.stack([sv.kbyu(np.eye(n[1])[j], 2) for j in range(n[1])], axis=1)
+    R = [np.stack([sv.restrict(np.eye(n[l])[j], l + 2) for j in range(n[l])], axis=1) for l in range(2)]
+    P = [np.stack([sv.interpolate(np.eye(n[l + 1])[j], l + 2) for j in range(n[l + 1])], axis=1) for l in range(2)]
+    S_ = [np.diag(w / m.levels[l].diagK) for l in range(2)]
+    M3 = np.linalg.inv(S.coarsest_matrix(m.levels[-1]).toarray())
+
+    def cycle(l, A, Mc):
+        I = np.eye(n[l])
+        if vtype == 'Adapted':
+            return 2 * S_[l] + P[l] @ Mc @ R[l]
+        return S_[l] + (I - S_[l] @ A) @ (S_[l] + P[l] @ Mc @ R[l] @ (I - A @ S_[l]))
+    M1 = cycle(0, A1, cycle(1, A2, M3))
+    r = np.random.default_rng(13).standard_normal(n[0])
+    ref = M1 @ r
+    got = sv.vcycle(r)
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    # the two cycle types are different operators on this mesh
+    other = S.Solver(m, vcycle_type='Adapted' if vtype == 'Standard' else 'Standard')
+    other.assemble(E)
+    assert np.abs(other.vcycle(r) - ref).max() > 1e-3 * np.abs(ref).max()
+
+
 def test_pcg_converges_to_direct_solve(setup):
     m, _, _ = setup
     E = 0.2 + 0.8 * np.random.default_rng(12).random(m.levels[0].numElements)

@@ -313,6 +313,71 @@ def test_adapted_vcycle_and_pcg_against_oracle(precision):
     torch.cuda.synchronize()
 
 
+@pytest.mark.parametrize('shape', [(28, 12, 12), (12, 12, 28), (16, 16, 8)], ids=['y_longest', 'z_longest', 'y_x_tie'])
+def test_coarse_plane_axes(shape):
+    """The coarsest solve orders its node planes along the coarsest grid's longest axis (y first on a
+    tie, then x); the other cases run along x.  Perforated cantilevers whose coarsest grid is longest in
+    y, longest in z, and tied between y and x: the coarsest solve, persistent and step by step, within
+    1e-9 of the oracle and 1e-10 of each other; the V-cycle (whose restriction packs the persistent
+    solve's right-hand side in plane order) within 1e-9 in FP64 and 2e-5 with the mixed mode's FP32
+    cycle.  FP64 PCG takes the oracle's iteration count (one more or fewer at most) and reaches its
+    compliance within 20x the oracle's own spread under a 2^-52 perturbation of b (the z-longest case
+    spreads by ~5e-8, the others by ~1e-15).  Mixed-mode PCG converges within twice the oracle's count and
+    1e-5 of its compliance: its FP32 V-cycle is a slightly inexact preconditioner, which costs CG more
+    iterations on the ill-conditioned z-longest case (51 against 36; 22 against 22 on y-longest)."""
+    import torch
+    from top_gpu_on_matlab_amd.model import Model
+    from top_gpu_on_matlab_amd.solver import Solver
+    vox = holey_mask(*shape, seed=5)
+    om = M.build_cuboid_model(*shape, opt_case=1, voxels=vox)
+    Es = smooth_modulus(om.levels[0])
+    osv = S.Solver(om)
+    osv.assemble(Es)
+    sv = {}
+    for p in ('fp64', 'mixed'):
+        gm = Model(vox, precision=p)
+        gm.builtin_case(1)
+        gm.finalize()
+        s = Solver(gm)
+        s.set_modulus(np.asarray(Es))
+        s.assemble()
+        sv[p] = s
+    g = sv['fp64']
+    rng = np.random.default_rng(14)
+    rc = rng.standard_normal(3 * om.levels[-1].numNodes)
+    ref = osv.coarse_solve(rc)
+    pers = g.coarse_solve(rc).cpu().numpy()
+    try:
+        g.set_tuning(coarse_persistent=False)
+        step = g.coarse_solve(rc).cpu().numpy()
+    finally:
+        g.set_tuning(coarse_persistent=True)
+    print('\n[%s] coarsest solve vs oracle: persistent %.1e, step %.1e' % (shape, _rel(pers, ref), _rel(step, ref)),
+          flush=True)
+    assert _rel(pers, ref) < 1e-9 and _rel(step, ref) < 1e-9 and _rel(pers, step) < 1e-10
+    r = rng.standard_normal(3 * om.levels[0].numNodes)
+    zref = osv.vcycle(r)
+    assert _rel(g.vcycle(r).cpu().numpy(), zref) < 1e-9
+    assert _rel(sv['mixed'].vcycle(r).cpu().numpy(), zref) < 2e-5
+    b = om.F[:, 0]
+    y_ref, its_ref, _ = osv.pcg(b, tol=1e-3, maxIT=800)
+    c_ref = float(b @ y_ref)
+    bp = b * (1 + 2.0 ** -52 * np.random.default_rng(11).standard_normal(b.shape))
+    c_spread = abs(float(b @ osv.pcg(bp, tol=1e-3, maxIT=800)[0]) - c_ref) / abs(c_ref)
+    for p, s in sv.items():
+        y, its, _ = s.pcg(b, tol=1e-3, max_it=800)
+        c_dev = abs(float(b @ y.cpu().numpy()) - c_ref) / abs(c_ref)
+        print('[%s %s] PCG its %d (oracle %d), compliance %.1e (oracle spread %.1e)'
+              % (shape, p, its, its_ref, c_dev, c_spread), flush=True)
+        if p == 'fp64':
+            assert abs(its - its_ref) <= 1
+            assert c_dev < max(20 * c_spread, 1e-12)
+        else:
+            assert its <= 2 * its_ref
+            assert c_dev < 1e-5
+    torch.cuda.synchronize()
+
+
 def test_reassembly_is_bitwise_reproducible():
     """Assembling the same moduli again must give bitwise the same hierarchy: coarse element
     matrices, Jacobi diagonals, coarsest factorisation (checked through its solve) and V-cycle.

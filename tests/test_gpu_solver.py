@@ -378,6 +378,47 @@ def test_coarse_plane_axes(shape):
     torch.cuda.synchronize()
 
 
+def test_coarse_solve_planes_wider_than_1024_dofs():
+    """A 144 x 144 x 144 cantilever: its coarsest grid has 18^3 elements, so a node plane holds 361
+    nodes, 1083 DOFs, more than the 1024 the persistent solve keeps in registers.  The solve then runs
+    step by step, with the solve operators' rows padded to 2048 and processed in 1024-wide chunks (every
+    other case has planes of at most 867 DOFs).  The oracle is too slow at this size, so the checks are:
+    the residual of the coarsest system, K_c x = r to 1e-10 (K_c x from the coarsest-level K*u, checked
+    against the oracle at small sizes); the captured-graph replay equal to the direct launches bit for
+    bit; PCG with the V-cycle around it reaching its tolerance, confirmed by the finest K*u."""
+    import torch
+    from top_gpu_on_matlab_amd.model import cuboid_model
+    from top_gpu_on_matlab_amd.solver import Solver
+    s = Solver(cuboid_model(144, 144, 144, 1, precision='fp64'))
+    L = s.num_levels
+    coarsest = s.levels[-1]
+    assert L == 3 and (coarsest['resY'], coarsest['resX'], coarsest['resZ']) == (18, 18, 18)
+    # a smooth density field, the kind an optimisation produces (an unfiltered random one makes the
+    # damped-Jacobi weight 0.6 too large, whatever the coarsest solve)
+    e = np.arange(144 ** 3)
+    ey, ex, ez = e % 144, (e // 144) % 144, e // 144 ** 2
+    x = 0.12 + 0.05 * np.sin(ex / 7.0) * np.cos(ey / 5.0) * np.sin(ez / 6.0 + 1)
+    s.set_modulus(1e-6 + x ** 3 * (1 - 1e-6))
+    s.assemble()
+    g = torch.Generator().manual_seed(3)
+    r = torch.randn(s.ndof(L), generator=g, dtype=torch.float64).cuda()
+    x = s.coarse_solve(r).clone()
+    res = float((s.kbyu(x, L) - r).norm() / r.norm())
+    try:
+        s.set_tuning(coarse_graph=True)
+        xg = s.coarse_solve(r).clone()
+    finally:
+        s.set_tuning(coarse_graph=False)
+    b = torch.randn(s.ndof(1), generator=g, dtype=torch.float64).cuda()
+    y, its, resnorm = s.pcg(b, tol=1e-6, max_it=400)
+    true_res = float((s.kbyu(y, 1) - b).norm() / b.norm())
+    print('\n[144^3, planes of 1083 DOFs] coarsest residual %.1e; PCG %d its, residual %.1e (reported %.1e)'
+          % (res, its, true_res, resnorm), flush=True)
+    assert res < 1e-10
+    assert xg.equal(x)
+    assert its < 400 and true_res < 1.5e-6
+
+
 def test_reassembly_is_bitwise_reproducible():
     """Assembling the same moduli again must give bitwise the same hierarchy: coarse element
     matrices, Jacobi diagonals, coarsest factorisation (checked through its solve) and V-cycle.

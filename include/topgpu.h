@@ -145,6 +145,11 @@ int tg_model_builtin_case(tg_model *m, int32_t opt_case);
  * loads [nLoad x 4] (node, Fx, Fy, Fz), both column-major double. */
 int tg_model_set_conditions(tg_model *m, const double *fixing, int64_t n_fix,
                             const double *loads, int64_t n_load);
+/* One more load case, [nLoad x 4] as above, after tg_model_set_conditions (a .TopVoxel file's further
+ * load blocks, loadingCond_{2..}, TOP_GPU.m:453-461): F_ gets a column per case, each case's loaded
+ * nodes leave the fixations as the first case's do (TOP_GPU.m:1511-1524), and the optimisation solves
+ * every case and weights them equally (objWeightingList_, TOP_GPU.m:155, 186, 461, 599). */
+int tg_model_add_loads(tg_model *m, const double *loads, int64_t n_load);
 /* FEA_ApplyBoundaryCondition + Solving_SetupKeWithFixedDOFs (TOP_GPU.m:1505-1727). */
 int tg_model_finalize(tg_model *m);
 
@@ -282,9 +287,11 @@ int tg_simp_modulus(const double *d_x, int64_t n, double Emin, double E0, double
 int tg_sensitivity(const double *d_x, const double *d_ce, int64_t n, int32_t per, double Emin, double E0,
                    double penal, double cSolid, double *d_dc, void *stream);
 
-enum { TG_EW_MUL = 0, TG_EW_DIV_FLOOR = 1, TG_EW_SET_WHERE = 2 };
+enum { TG_EW_MUL = 0, TG_EW_DIV_FLOOR = 1, TG_EW_SET_WHERE = 2, TG_EW_AXPY = 3 };
 /* out = a .* b (TG_EW_MUL) or a ./ max(scalar, b) (TG_EW_DIV_FLOOR)  (ft = 1 filtering, TOP_GPU.m:214);
- * out = (b ~= 0) ? scalar : a (TG_EW_SET_WHERE: xPhys(passiveElements) = 1, TOP_GPU.m:270, 273). */
+ * out = (b ~= 0) ? scalar : a (TG_EW_SET_WHERE: xPhys(passiveElements) = 1, TOP_GPU.m:270, 273);
+ * out = a + scalar .* b (TG_EW_AXPY, out may be a: the load cases' weighted unit compliances,
+ * ceList * objWeightingList_, TOP_GPU.m:599, 2644, 2671). */
 int tg_elementwise(int32_t op, const double *d_a, const double *d_b, double scalar, int64_t n, double *d_out,
                    void *stream);
 

@@ -402,8 +402,9 @@ def build_file_model(nelx, nely, nelz, voxels, fixing, loads, nu=0.3, coarsest_r
         return a
 
     fixing = prep(fixing)
-    loading = prep(loads)
-    fixing, ldl, F, fixed = apply_boundary_condition(finest, fixing, [loading])
+    # one [n x 4] block, or a list of them: the file's load cases (loadingCond_, TOP_GPU.m:447-461)
+    blocks = loads if isinstance(loads, (list, tuple)) else [loads]
+    fixing, ldl, F, fixed = apply_boundary_condition(finest, fixing, [prep(b) for b in blocks])
     finest.Ke = C.element_stiffness(nu, 1.0)
     mapU, free, fixK, eleFix = setup_ke_with_fixed_dofs(finest, fixing, finest.Ke, 1.0, False)
     return Model(levels=levels, numLevels=numLevels, nonDyadic=nd, nelx=nelx, nely=nely, nelz=nelz,

@@ -131,7 +131,8 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
     again after the density filter (:270, :273); in super-element mode only sub-voxel row Num_ = 4
     of each passive element is set (xnew(Num_, passiveElements), TOP_GPU.m:247, 273).
 
-    Returns dict with cHist, volHist, sharpHist, consHist, lssIts, cSolid, xPhys, U, times.
+    Returns dict with cHist, volHist, sharpHist, consHist, lssIts (every solve, load cases in turn),
+    cSolid, xPhys, U (the first load case's solution), Us (every case's), times.
     """
     pas = np.asarray(passive if passive is not None else [], dtype=np.int64)
     sv = S.Solver(model, weight_factor_jacobi=weight_factor_jacobi)
@@ -146,16 +147,28 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
         filt = ConvolutionFilter(model.nelx, model.nely, model.nelz, rMin, num)
         x = np.full((num ** 3, Ne), V0) if x_init is None else np.array(x_init, dtype=np.float64)
     xPhys = x.copy()
-    F = model.F[:, 0]
+    # one PCG solve per load case (TOP_GPU.m:155, 186), their unit compliances weighted equally
+    # (objWeightingList_, TOP_GPU.m:461; ceList * objWeightingList_, :599, 2644, 2671)
+    F = model.F
+    nC = F.shape[1]
+
+    def weigh(parts):
+        if nC == 1:
+            return parts[0]
+        out = np.zeros_like(parts[0])
+        for p in parts:
+            out = out + (1.0 / nC) * p
+        return out
+
     # fully solid reference (TOP_GPU.m:142-166)
     E1 = np.full((num ** 3, Ne), E0) if se else np.full(Ne, E0)
     sv.assemble(E1)
-    U, _, _ = sv.pcg(F, tol, maxIT)
+    Us = [sv.pcg(F[:, k], tol, maxIT)[0] for k in range(nC)]
     if not se:
-        ce = unit_compliance(lv1, U, lv1.Ke)
+        ce = weigh([unit_compliance(lv1, u, lv1.Ke) for u in Us])
         cSolid = float(E1 @ ce)
     else:
-        ce = unit_compliance_super(lv1, U, sv.K_mat)
+        ce = weigh([unit_compliance_super(lv1, u, sv.K_mat) for u in Us])
         cSolid = float(np.sum(ce))
 
     def simp(xp):
@@ -173,12 +186,13 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
         E = simp(xPhys)
         sv.assemble(E)
         t1 = time.perf_counter()
-        U, its, res = sv.pcg(F, tol, maxIT, y0=U)
+        for k in range(nC):  # each case warm-started from its previous solution
+            Us[k], its, res = sv.pcg(F[:, k], tol, maxIT, y0=Us[k])
+            out['lssIts'].append(its)
+            out['resTail'].append([float(v) for v in res[-2:]])   # relative residuals deciding the stop
         t2 = time.perf_counter()
-        out['lssIts'].append(its)
-        out['resTail'].append([float(v) for v in res[-2:]])   # relative residuals deciding the stop
         if not se:
-            ce = unit_compliance(lv1, U, lv1.Ke)
+            ce = weigh([unit_compliance(lv1, u, lv1.Ke) for u in Us])
             ceNorm = ce / cSolid
             cObj = float(E @ ceNorm)
             cDesign = cObj * cSolid
@@ -186,12 +200,12 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
             dc = -desimp(xPhys) * ceNorm
             dv = np.ones(Ne)
         else:
-            ce = unit_compliance_super(lv1, U, sv.K_mat)
+            ce = weigh([unit_compliance_super(lv1, u, sv.K_mat) for u in Us])
             ceNorm = ce / cSolid
             cObj = float(np.sum(ceNorm))
             cDesign = cObj * cSolid
             V = float(np.sum(xPhys) / (Ne * num ** 3))
-            dc = pre_compute_der(lv1, U, model.I_mat) / cSolid
+            dc = weigh([pre_compute_der(lv1, u, model.I_mat) for u in Us]) / cSolid
             dc = -desimp(xPhys) * dc
             dv = np.ones((num ** 3, Ne))
         t3 = time.perf_counter()
@@ -252,5 +266,6 @@ def run_top(model, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, Emin=1e-6, E0=1.0
                   % (loop, cDesign, V, sharp, fval, change, its))
     out['xPhys'] = xPhys
     out['x'] = x
-    out['U'] = U
+    out['U'] = Us[0]
+    out['Us'] = Us
     return out

@@ -116,6 +116,44 @@ def test_optimisation_loop_matches_oracle(cfg):
     assert _rel(got['xPhys'].cpu().numpy(), np.asarray(ref['xPhys']).reshape(-1, order='F')) < 1e-9
 
 
+def test_load_cases_optimisation_matches_oracle(tmp_path):
+    """A .TopVoxel cantilever with a second load case ('Additional Loads', TOP_GPU.m:451-461): every
+    iteration solves both cases, each warm-started from its own previous solution, and weights their unit
+    compliances equally (TOP_GPU.m:186, 599).  Against the oracle for 4 iterations.  Iteration 1 is the
+    same to 1e-10 (cSolid to 1e-12) with the same PCG counts for both cases.  After that CG amplifies
+    last-bit differences (the lateral case's residual ends near tol), so, as for C1, the compliances must
+    stay within 10x the deviation of the oracle whose loads are perturbed by 2^-52 (1e-10 to 5e-8 here;
+    the device 1e-9 to 2e-8) and every PCG count within max(2, 3x that run's count spread)."""
+    from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
+    from top_gpu_on_matlab_amd.topvoxel import read_topvoxel, write_topvoxel
+    nely, nelx, nelz = 8, 16, 8
+    vox = np.ones((nely, nelx, nelz), bool)
+    NY, NX = nely + 1, nelx + 1
+    node = lambda y, x, z: 1 + y + NY * (x + NX * z)
+    fix = np.array([[node(y, 0, z), 1, 1, 1] for z in range(nelz + 1) for y in range(nely + 1)], float)
+    A = np.array([[node(0, nelx, z), 0.0, 0.0, -1.0] for z in range(nelz + 1)])
+    B = np.array([[node(y, nelx, nelz // 2), 0.0, 1.0, 0.0] for y in range(nely + 1)])
+    path = os.path.join(str(tmp_path), 'two_loads.TopVoxel')
+    write_topvoxel(path, vox, fix, A, additional_loads=[B])
+    d = read_topvoxel(path)
+    om = M.build_file_model(nelx, nely, nelz, d['voxels'], d['fixing'], d['loads'])
+    ref = T.run_top(om, V0=0.12, rMin=math.sqrt(3), nLoop=4, ft=1)
+    om.F = om.F * (1 + 2.0 ** -52 * np.random.default_rng(1).standard_normal(om.F.shape))
+    up = T.run_top(om, V0=0.12, rMin=math.sqrt(3), nLoop=4, ft=1)
+    got = TOP_GPU(path, V0=0.12, rMin=math.sqrt(3), nLoop=4, ft=1)
+    dev = [abs(g - r) / r for g, r in zip(got['cHist'], ref['cHist'])]
+    spread = max(abs(u - r) / r for u, r in zip(up['cHist'], ref['cHist']))
+    its_spread = max(abs(u - r) for u, r in zip(up['lssIts'], ref['lssIts']))
+    print('\n[two load cases] its %s (oracle %s, perturbed %s), deviation %s (perturbed oracle up to %.1e)'
+          % (got['lssIts'], ref['lssIts'], up['lssIts'], ['%.1e' % v for v in dev], spread), flush=True)
+    assert len(got['lssIts']) == 2 * len(got['cHist']) == len(ref['lssIts'])
+    assert got['lssIts'][:2] == ref['lssIts'][:2]
+    assert abs(got['cSolid'] - ref['cSolid']) < 1e-12 * ref['cSolid']
+    assert dev[0] < 1e-10
+    assert max(dev) < max(10 * spread, 1e-10)
+    assert all(abs(a - b) <= max(2, 3 * its_spread) for a, b in zip(got['lssIts'], ref['lssIts']))
+
+
 def test_c1_fifty_iteration_history_against_golden():
     """TOP_GPU(true(24,48,24),'optCase',1,'V0',0.12): 50 iterations, FP64, against the oracle's history.
 

@@ -46,6 +46,31 @@ def test_topvoxel_round_trip_and_parity(tmp_path):
                                       [(1 % nely) + 16 * ((1 // nely) % nelx), (4 % nely) + 16 * ((4 // nely) % nelx)])
 
 
+def test_topvoxel_additional_load_cases(tmp_path):
+    """A .TopVoxel file with two additional load cases ('Additional Loads: 2', TOP_GPU.m:451-460): the
+    reader returns three load blocks, and the library's model has one load column per case, equal to
+    the oracle's (each case's loaded nodes leave the fixations, TOP_GPU.m:1511-1524)."""
+    from top_gpu_on_matlab_amd.topvoxel import load_topvoxel_model, read_topvoxel, write_topvoxel
+    nely, nelx, nelz = 9, 21, 11
+    vox = holey_mask(nely, nelx, nelz, seed=4)
+    fix, ld = _conditions(nely, nelx, nelz)
+    NY, NX = nely + 1, nelx + 1
+    node = lambda y, x, z: 1 + y + NY * (x + NX * z)
+    lateral = np.array([[node(y, nelx, z), 0.0, 1.0, 0.0] for y in range(3) for z in range(nelz + 1)])
+    axial = np.array([[node(nely, nelx, nelz), 1.0, 0.0, 0.5]])
+    path = os.path.join(tmp_path, 'loads.TopVoxel')
+    write_topvoxel(path, vox, fix, ld, additional_loads=[lateral, axial])
+    d = read_topvoxel(path)
+    assert len(d['loads']) == 3 and np.allclose(d['loads'][1], lateral) and np.allclose(d['loads'][2], axial)
+    gm = load_topvoxel_model(path)
+    om = M.build_file_model(nelx, nely, nelz, d['voxels'], d['fixing'], d['loads'])
+    F = gm.get_table(1, 'F')
+    assert F.shape == om.F.shape == (3 * om.levels[0].numNodes, 3)
+    assert np.array_equal(F, om.F)
+    assert np.array_equal(gm.get_table(1, 'fixedDOFs').astype(bool), om.fixedDOFs)
+    assert all(np.abs(F[:, k]).sum() > 0 for k in range(3))
+
+
 def test_nifti_and_history_writers(tmp_path):
     from top_gpu_on_matlab_amd.io import read_history, read_nifti1, write_histories, write_nifti1
     V = np.random.default_rng(0).random((4, 6, 5))

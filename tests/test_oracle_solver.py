@@ -343,3 +343,25 @@ def test_passive_elements_held_at_one(se):
         assert np.all(xp[pas] == 1.0)
         assert np.all(np.asarray(b['xPhys'])[pas] < 1.0)
     assert np.abs(np.array(a['cHist']) - np.array(b['cHist'])).max() > 1e-6 * abs(b['cHist'][-1])
+
+
+def test_load_cases_weighted_equally():
+    """Two load cases are solved one after the other and their unit compliances averaged
+    (objWeightingList_ = ones / 2, TOP_GPU.m:186, 461, 599): the fully solid compliance and the first
+    iteration's compliance are the means of the two single-case runs' (the same design, and each case's
+    PCG warm-started from its own solid solution), and the PCG counts are theirs in turn.  A sum
+    instead of the mean doubles them; a case left out gives one run's values."""
+    nely, nelx, nelz = 6, 12, 6
+    solid = np.arange(1, nely * nelx * nelz + 1)
+    NY, NX = nely + 1, nelx + 1
+    node = lambda y, x, z: 1 + y + NY * (x + NX * z)
+    fix = np.array([[node(y, 0, z), 1, 1, 1] for z in range(nelz + 1) for y in range(nely + 1)], float)
+    A = np.array([[node(0, nelx, z), 0.0, 0.0, -1.0] for z in range(nelz + 1)])
+    B = np.array([[node(y, nelx, nelz // 2), 0.0, 1.0, 0.0] for y in range(nely + 1)])
+    runs = {name: T.run_top(M.build_file_model(nelx, nely, nelz, solid, fix, loads), nLoop=1, ft=1)
+            for name, loads in (('A', A), ('B', B), ('AB', [A, B]))}
+    a, b, ab = runs['A'], runs['B'], runs['AB']
+    assert abs(ab['cSolid'] - 0.5 * (a['cSolid'] + b['cSolid'])) < 1e-12 * ab['cSolid']
+    assert abs(ab['cHist'][0] - 0.5 * (a['cHist'][0] + b['cHist'][0])) < 1e-12 * ab['cHist'][0]
+    assert ab['lssIts'] == [a['lssIts'][0], b['lssIts'][0]]
+    assert abs(a['cSolid'] - b['cSolid']) > 1e-3 * ab['cSolid']      # the cases differ

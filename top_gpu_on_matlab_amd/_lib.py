@@ -67,6 +67,7 @@ SIGNATURES = {
     'tg_model_destroy': (None, [_VP]),
     'tg_model_builtin_case': (ctypes.c_int, [_VP, _I32]),
     'tg_model_set_conditions': (ctypes.c_int, [_VP, _VP, _I64, _VP, _I64]),
+    'tg_model_add_loads': (ctypes.c_int, [_VP, _VP, _I64]),
     'tg_model_finalize': (ctypes.c_int, [_VP]),
     'tg_model_num_levels': (_I32, [_VP]),
     'tg_model_level_info': (ctypes.c_int, [_VP, _I32, ctypes.POINTER(TgLevelInfo)]),
@@ -108,7 +109,7 @@ SIGNATURES = {
     'tg_convfilter_forward': (ctypes.c_int, [_VP, _VP, _VP]),
     'tg_convfilter_adjoint': (ctypes.c_int, [_VP, _VP, _VP]),
 }
-TG_EW_MUL, TG_EW_DIV_FLOOR, TG_EW_SET_WHERE = 0, 1, 2
+TG_EW_MUL, TG_EW_DIV_FLOOR, TG_EW_SET_WHERE, TG_EW_AXPY = 0, 1, 2, 3
 (TG_TUNE_KFINE_KERNEL, TG_TUNE_KFINE_ZCHUNK, TG_TUNE_TIME_KFINE, TG_TUNE_KELEM_PARTS, TG_TUNE_COARSE_GRAPH,
  TG_TUNE_KFINE_MFMA, TG_TUNE_COARSE_PERSISTENT, TG_TUNE_COARSE_STREAMING, TG_TUNE_COARSE_POLLS) = \
     1, 2, 3, 4, 5, 6, 7, 8, 9

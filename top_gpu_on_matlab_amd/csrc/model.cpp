@@ -532,6 +532,14 @@ int tg_model_set_conditions(tg_model *m, const double *fixing, int64_t n_fix, co
     });
 }
 
+int tg_model_add_loads(tg_model *m, const double *loads, int64_t n_load) {
+    return guard("tg_model_add_loads:Input", [&] {
+        if (!m || (!loads && n_load) || n_load < 0) fail(TG_ERR_ARGS, "tg_model_add_loads:Input", "null argument");
+        m->loads.emplace_back(loads, loads + 4 * n_load);
+        m->finalized = false;
+    });
+}
+
 int tg_model_finalize(tg_model *m) {
     return guard("tg_model_finalize:Input", [&] {
         if (!m) fail(TG_ERR_ARGS, "tg_model_finalize:Input", "null model");

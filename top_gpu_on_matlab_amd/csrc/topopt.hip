@@ -108,6 +108,13 @@ __global__ void k_set_where(const double *__restrict__ a, const double *__restri
     if (i < n) c[i] = b[i] != 0.0 ? v : a[i];
 }
 
+// c = a + v b  (the load cases' weighted sum of unit compliances, ceList * objWeightingList_,
+// TOP_GPU.m:599, 2644, 2671); c may be a (no restrict: each entry is read before it is written)
+__global__ void k_axpy(const double *a, const double *__restrict__ b, double *c, double v, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c[i] = a[i] + v * b[i];
+}
+
 // ---- reductions ------------------------------------------------------------------------------
 enum { R_SUM = 0, R_DOT = 1, R_MAXABSDIFF = 2, R_SHARP = 3 };
 
@@ -582,6 +589,7 @@ int tg_elementwise(int32_t op, const double *d_a, const double *d_b, double scal
         if (op == TG_EW_MUL) k_mul<<<grid_for(n, 256), 256, 0, st>>>(d_a, d_b, d_out, n);
         else if (op == TG_EW_DIV_FLOOR) k_div_floor<<<grid_for(n, 256), 256, 0, st>>>(d_a, d_b, d_out, scalar, n);
         else if (op == TG_EW_SET_WHERE) k_set_where<<<grid_for(n, 256), 256, 0, st>>>(d_a, d_b, d_out, scalar, n);
+        else if (op == TG_EW_AXPY) k_axpy<<<grid_for(n, 256), 256, 0, st>>>(d_a, d_b, d_out, scalar, n);
         else fail(TG_ERR_ARGS, "tg_elementwise:Input", "unknown op %d", op);
         TG_LAUNCH_CHECK("tg_elementwise:HIP");
     });

@@ -66,6 +66,14 @@ class Model:
         check(lib().tg_model_set_conditions(self.h, fx.ctypes.data, fx.shape[0], ld.ctypes.data, ld.shape[0]))
         self.finalized = False
 
+    def add_loads(self, loads):
+        """One more load case [n x 4] (node, Fx, Fy, Fz) after set_conditions: a .TopVoxel file's further
+        load blocks (loadingCond_{2..}, TOP_GPU.m:453-461), solved and weighted equally by the
+        optimisation (objWeightingList_, TOP_GPU.m:461, 599)."""
+        ld = np.asfortranarray(np.asarray(loads, dtype=np.float64).reshape(-1, 4))
+        check(lib().tg_model_add_loads(self.h, ld.ctypes.data, ld.shape[0]))
+        self.finalized = False
+
     def finalize(self):
         check(lib().tg_model_finalize(self.h))
         self.finalized = True

@@ -149,6 +149,17 @@ def subvoxel_energy(solver, U):
     return dc
 
 
+def weigh_cases(parts):
+    """sum_k parts[k] / n over the load cases (ceList * objWeightingList_ with objWeightingList_ =
+    ones / n, TOP_GPU.m:461, 599, 2644, 2671), on the device; a single case as it is."""
+    if len(parts) == 1:
+        return parts[0]
+    out = torch.zeros_like(parts[0])
+    for p in parts:
+        out = elementwise(_lib.TG_EW_AXPY, out, p, 1.0 / len(parts))
+    return out
+
+
 def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, mixed_Precision=0,
             super_element=0, *, Emin=1e-6, E0=1.0, penal=3.0, nu=0.3, tol=1e-3, maxIT=800, x0=None,
             out_path=None, verbose=False, model=None, tuning=None, weightFactorJacobi=0.6):
@@ -192,7 +203,8 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
     else:
         filt = PDEFilter(model, rMin, tol=tol / 10, max_it=maxIT)
         shape = (Ne,)
-    F = torch.from_numpy(model.get_table(1, 'F')[:, 0].copy()).to(dev)
+    # one PCG solve per load case (TOP_GPU.m:155, 186)
+    Fs = [torch.from_numpy(np.ascontiguousarray(col)).to(dev) for col in model.get_table(1, 'F').T]
     if x0 is None:
         x = torch.full(shape, float(V0), dtype=torch.float64, device=dev)
     else:
@@ -203,8 +215,8 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
     E1 = torch.full(shape, E0, dtype=torch.float64, device=dev)
     solver.set_modulus(E1)
     solver.assemble()
-    U, _, _ = solver.pcg(F, tol=tol, max_it=maxIT)
-    ce = unit_compliance(solver, U)
+    Us = [solver.pcg(F, tol=tol, max_it=maxIT)[0] for F in Fs]
+    ce = weigh_cases([unit_compliance(solver, U) for U in Us])
     cSolid = reduce(_lib.TG_RED_DOT, E1, ce) if not se else reduce(_lib.TG_RED_SUM, ce)
     out = dict(cHist=[], volHist=[], sharpHist=[], consHist=[], lssIts=[], tHist=[], changeHist=[],
                cSolid=cSolid, pcg_ms=[], filter_its=[])
@@ -219,11 +231,12 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
         solver.assemble()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        U, its, _ = solver.pcg(F, y0=U, tol=tol, max_it=maxIT)
+        for k, F in enumerate(Fs):  # each case warm-started from its previous solution
+            Us[k], its, _ = solver.pcg(F, y0=Us[k], tol=tol, max_it=maxIT)
+            out['lssIts'].append(its)
+            out['pcg_ms'].append(solver.last_pcg_ms())
         t2 = time.perf_counter()
-        out['lssIts'].append(its)
-        out['pcg_ms'].append(solver.last_pcg_ms())
-        ce = unit_compliance(solver, U)
+        ce = weigh_cases([unit_compliance(solver, U) for U in Us])
         if not se:
             cObj = reduce(_lib.TG_RED_DOT, E, ce) / cSolid
             cDesign = cObj * cSolid
@@ -234,7 +247,7 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
             cObj = reduce(_lib.TG_RED_SUM, ce) / cSolid
             cDesign = cObj * cSolid
             V = reduce(_lib.TG_RED_SUM, xPhys) / (Ne * num ** 3)
-            dc = sensitivity(xPhys, subvoxel_energy(solver, U), cSolid, Emin, E0, penal)
+            dc = sensitivity(xPhys, weigh_cases([subvoxel_energy(solver, U) for U in Us]), cSolid, Emin, E0, penal)
             dv = ones
         t3 = time.perf_counter()
         fits = []
@@ -280,7 +293,8 @@ def TOP_GPU(inputModel, optCase=1, V0=0.12, rMin=math.sqrt(3), nLoop=50, ft=1, m
                   % (loop, cDesign, V, sharp, fval, change, its), flush=True)
     out['xPhys'] = xPhys
     out['x'] = x
-    out['U'] = U
+    out['U'] = Us[0]
+    out['Us'] = Us
     out['model'] = model
     if out_path:
         from .io import write_histories

@@ -84,9 +84,11 @@ def read_topvoxel(path):
                 loads=loads)
 
 
-def write_topvoxel(path, voxels_mask, fixing, loads, passive=(), densities=None):
+def write_topvoxel(path, voxels_mask, fixing, loads, passive=(), densities=None, additional_loads=()):
     """Write a .TopVoxel file.  voxels_mask: bool [nely, nelx, nelz]; fixing/loads in node indices of the
-    unpadded (nely+1)(nelx+1)(nelz+1) grid (1-based)."""
+    unpadded (nely+1)(nelx+1)(nelz+1) grid (1-based); additional_loads: further load cases, [n x 4] each,
+    written as the reference reads them (TOP_GPU.m:451-460: 'Additional Loads: k', then per case
+    'Loads: n index')."""
     nely, nelx, nelz = voxels_mask.shape
     solid = np.flatnonzero(np.asarray(voxels_mask).reshape(-1, order='F')) + 1
     with open(path, 'w') as f:
@@ -108,7 +110,11 @@ def write_topvoxel(path, voxels_mask, fixing, loads, passive=(), densities=None)
         f.write('Loads: %d\n' % len(loads))
         for r in loads:
             f.write('%d %.16e %.16e %.16e\n' % (int(r[0]), r[1], r[2], r[3]))
-        f.write('Additional Loads: 0\n')
+        f.write('Additional Loads: %d\n' % len(additional_loads))
+        for k, block in enumerate(additional_loads):
+            f.write('Loads: %d %d\n' % (len(block), k + 2))
+            for r in block:
+                f.write('%d %.16e %.16e %.16e\n' % (int(r[0]), r[1], r[2], r[3]))
 
 
 def _unique_sorted_rows(a):
@@ -144,10 +150,14 @@ def load_topvoxel_model(path, super_element=False, nu=0.3, precision='fp64', **k
     if len(fixing):
         fixing = fixing.copy()
         fixing[:, 0] = fwd[_adapt_nodes(fixing[:, 0], nely, nelx, nelz, m) - 1]
-    loads = _unique_sorted_rows(d['loads'][0]).copy()
-    if len(loads):
-        loads[:, 0] = fwd[_adapt_nodes(loads[:, 0], nely, nelx, nelz, m) - 1]
-    m.set_conditions(fixing, loads)
+    def prep(block):  # uniqued by node, sorted, moved to the padded grid, solid node numbers
+        loads = _unique_sorted_rows(block).copy()
+        if len(loads):
+            loads[:, 0] = fwd[_adapt_nodes(loads[:, 0], nely, nelx, nelz, m) - 1]
+        return loads
+    m.set_conditions(fixing, prep(d['loads'][0]))
+    for block in d['loads'][1:]:  # further load cases (TOP_GPU.m:453-461)
+        m.add_loads(prep(block))
     m.finalize()
     info = m.level_info(1)
     pas = np.sort(d['passive'])

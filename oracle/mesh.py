@@ -384,7 +384,7 @@ def adapt_bc_external(srcBC, nely, nelx, nelz, resX, resY, resZ):
 
 
 def build_file_model(nelx, nely, nelz, voxels, fixing, loads, nu=0.3, coarsest_resolution_control=20000,
-                     non_dyadic=1):
+                     non_dyadic=1, super_element=False):
     """CreateVoxelFEAmodel for a parsed .TopVoxel file (TOP_GPU.m:416-496): cellSize_ = 1 (TOP_GPU.m:67);
     fixations and loads are uniqued, sorted, moved to the padded grid and renumbered to solid nodes."""
     flat = np.zeros(nelx * nely * nelz, dtype=bool)
@@ -406,10 +406,16 @@ def build_file_model(nelx, nely, nelz, voxels, fixing, loads, nu=0.3, coarsest_r
     blocks = loads if isinstance(loads, (list, tuple)) else [loads]
     fixing, ldl, F, fixed = apply_boundary_condition(finest, fixing, [prep(b) for b in blocks])
     finest.Ke = C.element_stiffness(nu, 1.0)
-    mapU, free, fixK, eleFix = setup_ke_with_fixed_dofs(finest, fixing, finest.Ke, 1.0, False)
+    I_mat = None
+    if super_element:  # as for logical input (build_cuboid_model), with cellSize_ = 1
+        I_mat = C.super_element_matrices(4, 1.0, nu)
+        mapU, free, fixK, eleFix = setup_ke_with_fixed_dofs(finest, fixing, I_mat, 1.0, True)
+    else:
+        mapU, free, fixK, eleFix = setup_ke_with_fixed_dofs(finest, fixing, finest.Ke, 1.0, False)
     return Model(levels=levels, numLevels=numLevels, nonDyadic=nd, nelx=nelx, nely=nely, nelz=nelz,
                  fixing=fixing, loads=ldl, F=F, fixedDOFs=fixed, mapUniqueKes=mapU, uniqueKesFree=free,
-                 uniqueKesFixed=fixK, eleWithFixedDOFs=eleFix, cellSize=1.0, nu=nu)
+                 uniqueKesFixed=fixK, eleWithFixedDOFs=eleFix, cellSize=1.0, nu=nu,
+                 super_element=super_element, I_mat=I_mat)
 
 
 def build_cuboid_model(nely, nelx, nelz, opt_case=1, cell_size=None, nu=0.3, super_element=False,

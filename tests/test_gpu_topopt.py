@@ -154,6 +154,37 @@ def test_load_cases_optimisation_matches_oracle(tmp_path):
     assert all(abs(a - b) <= max(2, 3 * its_spread) for a, b in zip(got['lssIts'], ref['lssIts']))
 
 
+def test_load_cases_super_element_matches_oracle(tmp_path):
+    """The same two load cases on a 10 x 20 x 10 super-element box read from a .TopVoxel file (ft = 2, the
+    convolution filter): each case's sub-voxel energies are weighted equally as well (pre_compute_der,
+    TOP_GPU.m:2646-2671).  Iteration 1 to 1e-10 (cSolid to 1e-12) with the same PCG counts; iteration 2
+    within 10x the deviation of the oracle with its loads perturbed by 2^-52."""
+    from top_gpu_on_matlab_amd.top_gpu import TOP_GPU
+    from top_gpu_on_matlab_amd.topvoxel import read_topvoxel, write_topvoxel
+    nely, nelx, nelz = 10, 20, 10
+    NY, NX = nely + 1, nelx + 1
+    node = lambda y, x, z: 1 + y + NY * (x + NX * z)
+    fix = np.array([[node(y, 0, z), 1, 1, 1] for z in range(nelz + 1) for y in range(nely + 1)], float)
+    A = np.array([[node(0, nelx, z), 0.0, 0.0, -1.0] for z in range(nelz + 1)])
+    B = np.array([[node(y, nelx, nelz // 2), 0.0, 1.0, 0.0] for y in range(nely + 1)])
+    path = os.path.join(str(tmp_path), 'two_loads_se.TopVoxel')
+    write_topvoxel(path, np.ones((nely, nelx, nelz), bool), fix, A, additional_loads=[B])
+    d = read_topvoxel(path)
+    om = M.build_file_model(nelx, nely, nelz, d['voxels'], d['fixing'], d['loads'], super_element=True)
+    ref = T.run_top(om, V0=0.12, rMin=math.sqrt(3), nLoop=2, ft=2)
+    om.F = om.F * (1 + 2.0 ** -52 * np.random.default_rng(1).standard_normal(om.F.shape))
+    up = T.run_top(om, V0=0.12, rMin=math.sqrt(3), nLoop=2, ft=2)
+    got = TOP_GPU(path, V0=0.12, rMin=math.sqrt(3), nLoop=2, ft=2, super_element=1)
+    dev = [abs(g - r) / r for g, r in zip(got['cHist'], ref['cHist'])]
+    spread = max(abs(u - r) / r for u, r in zip(up['cHist'], ref['cHist']))
+    print('\n[two load cases, super-element] its %s (oracle %s), deviation %s (perturbed oracle up to %.1e)'
+          % (got['lssIts'], ref['lssIts'], ['%.1e' % v for v in dev], spread), flush=True)
+    assert len(got['lssIts']) == 4 and got['lssIts'][:2] == ref['lssIts'][:2]
+    assert abs(got['cSolid'] - ref['cSolid']) < 1e-12 * ref['cSolid']
+    assert dev[0] < 1e-10 and max(dev) < max(10 * spread, 1e-10)
+    assert all(abs(a - b) <= 2 for a, b in zip(got['lssIts'], ref['lssIts']))
+
+
 def test_c1_fifty_iteration_history_against_golden():
     """TOP_GPU(true(24,48,24),'optCase',1,'V0',0.12): 50 iterations, FP64, against the oracle's history.
 
